@@ -1,0 +1,146 @@
+/*
+ * sc.h -- C-ABI of libsc, the MI355X (gfx950) molecular-dynamics hot path of
+ * SimulationCode.jl.
+ *
+ * Plain pointers and sizes only.  Host arrays use the reference's host layout:
+ * Julia's Vector{SVector{dims,Float64}} is a contiguous AoS block of
+ * N x dims doubles ([x1 y1 z1 x2 y2 z2 ...], src/DataStructs.jl:183-201);
+ * diameters are N doubles.  Host arrays are borrowed for the duration of a
+ * call only (copied synchronously); the library keeps no host pointers.
+ * Device state is owned by the context and kept resident between calls.
+ *
+ * Every entry point returns an int status (SC_OK == 0, negative on error);
+ * sc_last_error() gives the text.  A context is not thread-safe; independent
+ * contexts are.  Each context owns one HIP stream (default priority).
+ *
+ * Reference entry points each function replaces are cited as file:line
+ * relative to the SimulationCode.jl checkout.
+ */
+#ifndef SC_H
+#define SC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SC_POT_LJ = 1, SC_POT_BERTHIER = 2 };
+enum { SC_SYS_NEWTONIAN = 0, SC_SYS_LANGEVIN = 1, SC_SYS_BROWNIAN = 2 };
+/* SC_PREC_FP64: all arithmetic in FP64 (the drop-in).
+ * SC_PREC_MIXED: pair distance and force arithmetic in FP32; positions,
+ * velocities, minimum-image wrapping, integrator updates and energy
+ * reductions stay FP64. */
+enum { SC_PREC_FP64 = 0, SC_PREC_MIXED = 1 };
+
+enum {
+    SC_OK = 0,
+    SC_ERR_ARG = -1,       /* bad argument / unsupported combination        */
+    SC_ERR_OOM = -2,       /* device allocation failed                      */
+    SC_ERR_OVERFLOW = -3,  /* a full-list row exceeded maxneighbours        */
+    SC_ERR_NONFINITE = -4, /* a position became NaN/Inf                     */
+    SC_ERR_HIP = -5,       /* HIP runtime error                             */
+    SC_ERR_STATE = -6      /* call out of order (e.g. no neighbour lists)   */
+};
+
+/* ParameterStruct + System + InteractionPotential + the scalar arguments of
+ * initialize_neighbor_struct (src/DataStructs.jl:10-19, 45-87, 138-153,
+ * 349-354).  Field meanings follow the reference. */
+typedef struct sc_params {
+    int32_t dims;                 /* 2 or 3 */
+    int32_t potential;            /* SC_POT_* */
+    int32_t system;               /* SC_SYS_* */
+    int32_t precision;            /* SC_PREC_* */
+    int32_t rescale_temperature;  /* Newtonian.rescale_temperature */
+    int32_t cell_divisor;         /* cells of side >= R_v / cell_divisor; 0 => 1 */
+    int64_t N;
+    int64_t maxneighbours;        /* initialize_neighbor_struct kwarg (200) */
+    double box_size;              /* ParameterStruct.box_size */
+    double force_cutoff;          /* initialize_neighbor_struct force_cutoff */
+    double force_cutoff2;         /* ParameterStruct.force_cutoff2 */
+    double skin_distance;         /* initialize_neighbor_struct skin_distance */
+    double lj_eps, lj_sigma, lj_Uc;          /* LJ (DataStructs.jl:10-19) */
+    double b_c0, b_c2, b_c4, b_zeta;          /* Berthier (DataStructs.jl:45-60) */
+    double kBT, gamma, dt, m;                 /* System (DataStructs.jl:64-87) */
+    uint64_t seed;                /* on-device Philox4x32-10 noise seed */
+} sc_params;
+
+typedef struct sc_ctx sc_ctx;
+
+/* initialize_neighbor_struct(skin, box_size, force_cutoff, N, D, dims;
+ * maxneighbours) -- src/DataStructs.jl:349-378.  Allocates the device
+ * state for one system; R_v = skin + force_cutoff * max(D). */
+int sc_create(sc_ctx **out, const sc_params *p, const double *D);
+void sc_destroy(sc_ctx *ctx);
+const char *sc_last_error(const sc_ctx *ctx);
+
+/* Upload/download ArrayStruct fields (src/DataStructs.jl:183-201) and the
+ * NeighborListStruct displacement arrays (:252-253).  Any pointer may be
+ * NULL (field untouched).  Setting r does not rebuild the lists, exactly as
+ * assigning arrays.r_array in the reference does not. */
+int sc_set_state(sc_ctx *ctx, const double *r, const double *v, const double *F,
+                 const double *D, const double *disp);
+int sc_get_state(sc_ctx *ctx, double *r, double *v, double *F, double *disp,
+                 double *disp2);
+
+/* update_neighbor_lists! -- src/NeighborLists.jl:24-53 (counts a rebuild,
+ * zeroes the displacements, rebuilds the Verlet lists on the device). */
+int sc_update_neighbor_lists(sc_ctx *ctx);
+
+/* Reference-layout lists for parity and for the CPU Monte-Carlo path
+ * (NeighborListStruct fields, src/DataStructs.jl:248-251): counts of length
+ * N, lists column-major maxneigh x N, 1-based indices.  Full rows hold every
+ * j with d2 <= R_v^2; half rows hold each pair with
+ * d2 < (sqrt(force_cutoff2)*sigma_ij + skin)^2 once (in the row of the smaller
+ * index).  Entry order within a row is unspecified, as in the reference.
+ * Any output may be NULL. */
+int sc_get_neighbor_lists(sc_ctx *ctx, int64_t maxneigh, int64_t *nfull, int64_t *full,
+                          int64_t *nhalf, int64_t *half);
+
+/* calculate_new_forces! -- src/MD.jl:262-316, including
+ * add_new_random_force! (:30-82).  noise: N x dims AoS standard normals
+ * (what randn! would have produced), or NULL for on-device Philox noise. */
+int sc_calculate_new_forces(sc_ctx *ctx, const double *noise);
+
+/* nsteps calls of do_time_step -- src/MD.jl:499-554 (Newtonian/Langevin
+ * velocity Verlet with COM removal, optional rescale, skin-triggered
+ * rebuild) and the intended Brownian step of src/MD.jl:429-471.  noise:
+ * nsteps consecutive N x dims AoS blocks, or NULL for on-device Philox.
+ * The state is reference-consistent when the call returns. */
+int sc_do_time_steps(sc_ctx *ctx, int64_t nsteps, const double *noise);
+
+/* calculate_full_energy -- src/AuxiliaryFunctions.jl:22-34 (+ find_energy
+ * :546-584): sum over full-list pairs, halved; compensated reduction. */
+int sc_full_energy(sc_ctx *ctx, double *out);
+/* calculate_full_energy_no_neigh -- src/AuxiliaryFunctions.jl:86-110: the
+ * O(N^2) all-pairs energy, tiled on the device. */
+int sc_energy_no_neigh(sc_ctx *ctx, double *out);
+/* find_kinetic_energy -- src/AuxiliaryFunctions.jl:57-65. */
+int sc_kinetic_energy(sc_ctx *ctx, double *out);
+/* rescale_velocities! -- src/MD.jl:391-402. */
+int sc_rescale_velocities(sc_ctx *ctx);
+
+/* OutputStruct.N_neighbor_list_rebuilds (src/DataStructs.jl:299): read and
+ * optionally overwrite (pass set >= 0). */
+int sc_rebuild_count(sc_ctx *ctx, int64_t *out, int64_t set);
+
+/* find_inherent_structure's capped steepest-descent step
+ * (src/InherentStructure.jl:26-51): `iters` iterations of
+ * r += min(1, max_step/max|F*dt|) * F*dt with the skin-triggered rebuild.
+ * Returns the final max |F| in *maxforce (may be NULL). */
+int sc_minimize(sc_ctx *ctx, int64_t iters, double dt, double max_step, double *maxforce);
+
+/* The on-device noise generator's counter-based block function
+ * (Philox4x32-10, Salmon et al., SC'11), exported so its known-answer
+ * vectors can be checked on the host.  No reference counterpart: the
+ * reference draws noise from Julia's global RNG (src/MD.jl:4-6). */
+void sc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Device-side wall time of the last sc_do_time_steps call (HIP events on the
+ * context stream), in milliseconds. */
+int sc_last_elapsed_ms(sc_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SC_H */

@@ -1,0 +1,538 @@
+/*
+ * sc_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, FP64, single-threaded restatement of SimulationCode.jl's
+ * molecular-dynamics hot path.  It is the checker for the HIP library and the
+ * CPU baseline ("port") of the benchmark.  Nothing in the product links it:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (see oracle/Makefile).
+ * No FMA contraction anywhere, so every expression rounds exactly as written.
+ *
+ * Every function cites the reference file:line it follows (paths relative to
+ * the reference checkout).  Julia semantics that matter for rounding:
+ *   round(x)      -> rint(x)   (RoundNearest = ties-to-even)
+ *   x^2           -> x*x       (literal_pow)
+ *   a*b*c         -> (a*b)*c   (n-ary * folds left)
+ *   sum(SVector)  -> left fold ((x1+x2)+x3)
+ *
+ * Parity notes (also in the Python wrapper header):
+ *   - CellListMap's internal pair order and d2 are not pinned (the package is
+ *     an unversioned dependency absent here); pairs are enumerated here with
+ *     calculate_squared_distance and the add_pair! predicates, which is what
+ *     every other reference routine uses.
+ *   - Rows are filled in ascending (i<j) visiting order; the half-list entry
+ *     goes into row min(i,j).  Orientation/order in the reference are
+ *     CellListMap internals; compare lists as sets.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_POT_LJ 1
+#define OR_POT_BERTHIER 2
+#define OR_SYS_NEWTONIAN 0
+#define OR_SYS_LANGEVIN 1
+#define OR_SYS_BROWNIAN 2
+
+#define OR_OK 0
+#define OR_ERR_OVERFLOW -1
+#define OR_ERR_ARG -2
+
+/* Mirrors ParameterStruct + System + InteractionPotential fields the hot path
+ * reads (src/DataStructs.jl:10-19, 45-87, 138-153) plus the neighbour-struct
+ * scalars (src/DataStructs.jl:243-257, 349-354). */
+typedef struct {
+    int32_t dims;
+    int32_t potential;          /* OR_POT_* */
+    int32_t system;             /* OR_SYS_* */
+    int32_t rescale_temperature;
+    int64_t N;
+    int64_t maxneighbours;
+    double box_size;
+    double force_cutoff2;
+    double skin_distance;
+    double verlet_sphere_size;  /* skin + force_cutoff*max(D) */
+    /* LJ */
+    double lj_eps, lj_sigma, lj_Uc;
+    /* Berthier */
+    double b_c0, b_c2, b_c4, b_zeta;
+    /* dynamics */
+    double kBT, gamma, dt, m;
+} or_params;
+
+/* src/DataStructs.jl:15-18 */
+double or_lj_Uc(double eps, double sigma, double rc)
+{
+    double q = sigma / rc;
+    return 4.0 * eps * (pow(q, 12.0) - pow(q, 6.0));
+}
+
+/* src/DataStructs.jl:53-58 (n = 12 asserted at :54) */
+void or_berthier_constants(double rc, double out[3])
+{
+    double n = 12.0;
+    out[0] = -(1.0 / 8.0) * (8.0 + 6.0 * n + n * n) * pow(rc, -n);
+    out[1] = 1.0 / 4.0 * n * (4.0 + n) * pow(rc, -2.0 - n);
+    out[2] = -(1.0 / 8.0) * n * (2.0 + n) * pow(rc, -4.0 - n);
+}
+
+/* src/AuxiliaryFunctions.jl:311-315 */
+double or_squared_distance(const double *ri, const double *rj, int dims, double L)
+{
+    double s = 0.0;
+    for (int c = 0; c < dims; c++) {
+        double dr = ri[c] - rj[c];
+        dr -= rint(dr / L) * L;
+        double sq = dr * dr;
+        s = (c == 0) ? sq : s + sq;
+    }
+    return s;
+}
+
+/* src/InteractionPotential.jl:107-117 */
+static inline double mean_d(const or_params *p, double Di, double Dj)
+{
+    if (p->potential == OR_POT_LJ)
+        return p->lj_sigma * p->lj_sigma;          /* quirk: sigma^2 */
+    return (Di + Dj) * 0.5 * (1.0 - p->b_zeta * fabs(Di - Dj));
+}
+
+/* src/InteractionPotential.jl:123-131 (LJ), :193-201 (Berthier) */
+static inline double potential(const or_params *p, double r2, double md2)
+{
+    if (p->potential == OR_POT_LJ) {
+        double xi2 = md2 / r2;
+        double xi4 = xi2 * xi2;
+        double xi6 = xi2 * xi4;
+        double xi12 = xi6 * xi6;
+        return 4.0 * p->lj_eps * (xi12 - xi6) - p->lj_Uc;
+    }
+    double invxi2 = r2 / md2;
+    double invxi4 = invxi2 * invxi2;
+    double xi12 = 1.0 / (invxi4 * invxi4 * invxi4);
+    return xi12 + p->b_c4 * invxi4 + p->b_c2 * invxi2 + p->b_c0;
+}
+
+/* src/InteractionPotential.jl:145-152 (LJ), :204-213 (Berthier) */
+static inline double force_divr(const or_params *p, double r2, double md2)
+{
+    if (p->potential == OR_POT_LJ) {
+        double inv_r2 = 1.0 / r2;
+        double xi2 = md2 * inv_r2;
+        double xi6 = xi2 * xi2 * xi2;
+        double xi12 = xi6 * xi6;
+        return p->lj_eps * (48.0 * xi12 - 24.0 * xi6) * inv_r2;
+    }
+    double inv_md2 = 1.0 / md2;
+    double invxi2 = r2 * inv_md2;
+    double invxi4 = invxi2 * invxi2;
+    double xi14 = 1.0 / (invxi4 * invxi4 * invxi4 * invxi2);
+    return -2.0 * inv_md2 * (p->b_c2 + 2.0 * p->b_c4 * invxi2 - 6.0 * xi14);
+}
+
+/* ------------------------------------------------------------------ lists */
+
+/* src/NeighborLists.jl:82-104 (add_pair!).  Lists are row-per-particle,
+ * 0-based: list[i*maxneigh + k]. */
+static int add_pair(const or_params *p, int64_t i, int64_t j, double d2,
+                    const double *D, int64_t *nfull, int64_t *full,
+                    int64_t *nhalf, int64_t *half)
+{
+    int64_t M = p->maxneighbours;
+    double r_cutoff = sqrt(p->force_cutoff2);          /* NeighborLists.jl:44 */
+    if (nfull[i] >= M || nfull[j] >= M)
+        return OR_ERR_OVERFLOW;
+    full[i * M + nfull[i]++] = j;
+    double md = mean_d(p, D[i], D[j]);
+    double rh = r_cutoff * md + p->skin_distance;
+    if (d2 < rh * rh) {
+        if (nhalf[i] >= M)
+            return OR_ERR_OVERFLOW;
+        half[i * M + nhalf[i]++] = j;
+    }
+    full[j * M + nfull[j]++] = i;
+    return OR_OK;
+}
+
+/* src/NeighborLists.jl:24-53 with CellListMap's map_pairwise! restated as an
+ * O(N^2) sweep over unordered pairs with d2 <= R_v^2. Zeroes the
+ * displacement arrays (NeighborLists.jl:35-42) if they are given. */
+int or_build_lists_brute(const or_params *p, const double *r, const double *D,
+                         int64_t *nfull, int64_t *full, int64_t *nhalf, int64_t *half,
+                         double *disp, double *disp2)
+{
+    int64_t N = p->N;
+    int dims = p->dims;
+    double Rv = p->verlet_sphere_size;
+    double Rv2 = Rv * Rv;
+    memset(nfull, 0, sizeof(int64_t) * N);
+    memset(nhalf, 0, sizeof(int64_t) * N);
+    if (disp) memset(disp, 0, sizeof(double) * N * dims);
+    if (disp2) memset(disp2, 0, sizeof(double) * N);
+    for (int64_t i = 0; i < N; i++) {
+        for (int64_t j = i + 1; j < N; j++) {
+            double d2 = or_squared_distance(r + i * dims, r + j * dims, dims, p->box_size);
+            if (d2 <= Rv2) {
+                int rc = add_pair(p, i, j, d2, D, nfull, full, nhalf, half);
+                if (rc) return rc;
+            }
+        }
+    }
+    return OR_OK;
+}
+
+/* Same pair set through a serial cell list (cells of side >= R_v, 3^dims
+ * stencil); the CPU-baseline path for large N.  Falls back to the O(N^2)
+ * sweep when fewer than 3 cells fit per side. */
+int or_build_lists_cells(const or_params *p, const double *r, const double *D,
+                         int64_t *nfull, int64_t *full, int64_t *nhalf, int64_t *half,
+                         double *disp, double *disp2)
+{
+    int64_t N = p->N;
+    int dims = p->dims;
+    double L = p->box_size;
+    double Rv = p->verlet_sphere_size;
+    double Rv2 = Rv * Rv;
+    int64_t nc = (int64_t)floor(L / Rv);
+    if (nc < 3)
+        return or_build_lists_brute(p, r, D, nfull, full, nhalf, half, disp, disp2);
+    double cs = L / (double)nc;
+    int64_t ncell = nc * nc * (dims == 3 ? nc : 1);
+    int64_t *head = (int64_t *)malloc(sizeof(int64_t) * ncell);
+    int64_t *next = (int64_t *)malloc(sizeof(int64_t) * N);
+    int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * N);
+    if (!head || !next || !cell) { free(head); free(next); free(cell); return OR_ERR_ARG; }
+    for (int64_t c = 0; c < ncell; c++) head[c] = -1;
+    memset(nfull, 0, sizeof(int64_t) * N);
+    memset(nhalf, 0, sizeof(int64_t) * N);
+    if (disp) memset(disp, 0, sizeof(double) * N * dims);
+    if (disp2) memset(disp2, 0, sizeof(double) * N);
+    /* build linked cells; insert in descending i so each chain is ascending */
+    for (int64_t i = N - 1; i >= 0; i--) {
+        int64_t ci[3] = {0, 0, 0};
+        for (int c = 0; c < dims; c++) {
+            int64_t k = (int64_t)floor(r[i * dims + c] / cs);
+            if (k < 0) k = 0;
+            if (k >= nc) k = nc - 1;
+            ci[c] = k;
+        }
+        int64_t id = ci[0] + nc * (ci[1] + nc * ci[2]);
+        cell[i] = id;
+        next[i] = head[id];
+        head[id] = i;
+    }
+    int rc = OR_OK;
+    for (int64_t i = 0; i < N && rc == OR_OK; i++) {
+        int64_t id = cell[i];
+        int64_t cx = id % nc, cy = (id / nc) % nc, cz = id / (nc * nc);
+        int zr = (dims == 3) ? 1 : 0;
+        for (int dz = -zr; dz <= zr && rc == OR_OK; dz++)
+            for (int dy = -1; dy <= 1 && rc == OR_OK; dy++)
+                for (int dx = -1; dx <= 1 && rc == OR_OK; dx++) {
+                    int64_t nx = (cx + dx + nc) % nc, ny = (cy + dy + nc) % nc;
+                    int64_t nz = (dims == 3) ? (cz + dz + nc) % nc : 0;
+                    int64_t nid = nx + nc * (ny + nc * nz);
+                    for (int64_t j = head[nid]; j >= 0; j = next[j]) {
+                        if (j <= i) continue;
+                        double d2 = or_squared_distance(r + i * dims, r + j * dims, dims, L);
+                        if (d2 <= Rv2) {
+                            rc = add_pair(p, i, j, d2, D, nfull, full, nhalf, half);
+                            if (rc) break;
+                        }
+                    }
+                }
+    }
+    free(head); free(next); free(cell);
+    return rc;
+}
+
+/* ----------------------------------------------------------------- forces */
+
+/* src/MD.jl:85-137 (Berthier 3D), :140-191 (LJ 3D), :194-238 (Berthier 2D),
+ * :291-310 (generic path: 2D LJ).  Half-list gather with Newton-3rd scatter.
+ * F must be zeroed by the caller path (or_calculate_new_forces). */
+static void forces_pairs(const or_params *p, const double *r, const double *D,
+                         const int64_t *nhalf, const int64_t *half, double *F)
+{
+    int64_t N = p->N, M = p->maxneighbours;
+    int dims = p->dims;
+    double L = p->box_size, fc2 = p->force_cutoff2;
+    for (int64_t i = 0; i < N; i++) {
+        int64_t nn = nhalf[i];
+        if (nn == 0) continue;                               /* MD.jl:280-283 */
+        const double *ri = r + i * dims;
+        double Di = D[i];
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int64_t k = 0; k < nn; k++) {
+            int64_t j = half[i * M + k];
+            const double *rj = r + j * dims;
+            double dr[3] = {0.0, 0.0, 0.0};
+            for (int c = 0; c < dims; c++) {
+                dr[c] = ri[c] - rj[c];
+                dr[c] -= rint(dr[c] / L) * L;
+            }
+            double rij2 = dr[0] * dr[0] + dr[1] * dr[1];
+            if (dims == 3) rij2 = rij2 + dr[2] * dr[2];
+            double Fs;
+            if (p->potential == OR_POT_BERTHIER) {           /* MD.jl:120-126 */
+                double Dj = D[j];
+                double md = (Di + Dj) * 0.5 * (1.0 - p->b_zeta * fabs(Di - Dj));
+                double md2 = md * md;
+                double inv_md2 = 1.0 / md2;
+                double invxi2 = rij2 * inv_md2;
+                double invxi4 = invxi2 * invxi2;
+                double xi14 = 1.0 / (invxi4 * invxi4 * invxi4 * invxi2);
+                Fs = (fc2 * md2 < rij2) ? 0.0
+                     : -2.0 * inv_md2 * (p->b_c2 + 2.0 * p->b_c4 * invxi2 - 6.0 * xi14);
+            } else if (dims == 3) {                          /* MD.jl:175-180 */
+                double s2 = p->lj_sigma * p->lj_sigma;
+                double xi2 = s2 / rij2;
+                double xi6 = xi2 * xi2 * xi2;
+                double xi12 = xi6 * xi6;
+                Fs = (fc2 < rij2) ? 0.0 : p->lj_eps * (48.0 * xi12 - 24.0 * xi6) / rij2;
+            } else {                                         /* MD.jl:299-307 */
+                double md = mean_d(p, Di, D[j]);
+                double md2 = md * md;
+                Fs = (fc2 * md2 > rij2) ? force_divr(p, rij2, md2) : 0.0;
+            }
+            for (int c = 0; c < dims; c++) {
+                f[c] += Fs * dr[c];
+                F[j * dims + c] -= Fs * dr[c];
+            }
+        }
+        for (int c = 0; c < dims; c++)
+            F[i * dims + c] += f[c];
+    }
+}
+
+/* src/MD.jl:262-316 calculate_new_forces! followed by add_new_random_force!
+ * (src/MD.jl:30-43 Langevin, :67-78 Brownian, :80-82 Newtonian).  `noise`
+ * is the N x dims AoS block randn! would have produced (may be NULL for
+ * Newtonian). */
+void or_calculate_new_forces(const or_params *p, const double *r, const double *D,
+                             const int64_t *nhalf, const int64_t *half,
+                             const double *v, const double *noise, double *F)
+{
+    int64_t N = p->N;
+    int dims = p->dims;
+    memset(F, 0, sizeof(double) * N * dims);
+    forces_pairs(p, r, D, nhalf, half, F);
+    if (p->system == OR_SYS_NEWTONIAN || noise == NULL)
+        return;
+    double factor = sqrt(2.0 * p->kBT * p->gamma / p->dt);
+    for (int64_t i = 0; i < N; i++)
+        for (int c = 0; c < dims; c++) {
+            double add;
+            if (p->system == OR_SYS_LANGEVIN)
+                add = -p->gamma * v[i * dims + c] + factor * noise[i * dims + c];
+            else
+                add = factor * noise[i * dims + c];
+            F[i * dims + c] = F[i * dims + c] + add;
+        }
+}
+
+/* ---------------------------------------------------------------- energy */
+
+/* src/AuxiliaryFunctions.jl:546-584 (find_energy) with the specialised inner
+ * loops :381-413 (Berthier 3D), :416-444 (Berthier 2D), :477-507 (LJ 3D) and
+ * the generic loop :568-581 (2D LJ). */
+static double find_energy(const or_params *p, int64_t i, const double *r, const double *D,
+                          const int64_t *nfull, const int64_t *full)
+{
+    int64_t nn = nfull[i], M = p->maxneighbours;
+    int dims = p->dims;
+    double L = p->box_size, fc2 = p->force_cutoff2;
+    double e = 0.0;
+    if (nn == 0) return 0.0;
+    const double *ri = r + i * dims;
+    double Di = D[i];
+    int generic = (p->potential == OR_POT_LJ && dims == 2);
+    for (int64_t k = 0; k < nn; k++) {
+        int64_t j = full[i * M + k];
+        double rij2 = or_squared_distance(ri, r + j * dims, dims, L);
+        double md, md2;
+        if (p->potential == OR_POT_LJ && !generic) {
+            md = p->lj_sigma;                       /* LJ_inner_loop: mean_d = σ */
+            md2 = md * md;
+        } else {
+            md = mean_d(p, Di, D[j]);
+            md2 = md * md;
+        }
+        if (generic) {
+            if (fc2 * md2 > rij2) e += potential(p, rij2, md2);
+        } else {
+            double v;
+            if (p->potential == OR_POT_LJ) {        /* :498-504 */
+                double xi2 = md2 / rij2;
+                double xi6 = xi2 * xi2 * xi2;
+                double xi12 = xi6 * xi6;
+                v = 4.0 * p->lj_eps * (xi12 - xi6) - p->lj_Uc;
+            } else {
+                v = potential(p, rij2, md2);
+            }
+            e += (fc2 * md2 < rij2) ? 0.0 : v;
+        }
+    }
+    return e;
+}
+
+/* src/AuxiliaryFunctions.jl:22-34 */
+double or_full_energy(const or_params *p, const double *r, const double *D,
+                      const int64_t *nfull, const int64_t *full)
+{
+    double e = 0.0;
+    for (int64_t i = 0; i < p->N; i++)
+        e += find_energy(p, i, r, D, nfull, full);
+    return e / 2.0;
+}
+
+/* src/AuxiliaryFunctions.jl:86-110 (O(N^2), generic potential()) */
+double or_energy_no_neigh(const or_params *p, const double *r, const double *D)
+{
+    int dims = p->dims;
+    double e = 0.0;
+    for (int64_t i = 0; i < p->N; i++)
+        for (int64_t j = 0; j < p->N; j++) {
+            if (i == j) continue;
+            double rij2 = or_squared_distance(r + i * dims, r + j * dims, dims, p->box_size);
+            double md = mean_d(p, D[i], D[j]);
+            double md2 = md * md;
+            e += (p->force_cutoff2 * md2 < rij2) ? 0.0 : potential(p, rij2, md2);
+        }
+    return e / 2.0;
+}
+
+/* src/AuxiliaryFunctions.jl:57-65 */
+double or_kinetic_energy(const or_params *p, const double *v)
+{
+    int dims = p->dims;
+    double ek = 0.0;
+    for (int64_t i = 0; i < p->N; i++) {
+        double s = v[i * dims] * v[i * dims];
+        for (int c = 1; c < dims; c++) s = s + v[i * dims + c] * v[i * dims + c];
+        ek += s;
+    }
+    return ek * (p->m / 2.0);
+}
+
+/* src/MD.jl:391-402 */
+void or_rescale_velocities(const or_params *p, double *v)
+{
+    int dims = p->dims;
+    double ek = 0.0;
+    for (int64_t i = 0; i < p->N; i++) {
+        double s = v[i * dims] * v[i * dims];
+        for (int c = 1; c < dims; c++) s = s + v[i * dims + c] * v[i * dims + c];
+        ek += s;
+    }
+    ek *= p->m / 2.0;
+    double kT = 2.0 * ek / ((double)dims * (double)p->N);
+    double factor = sqrt(p->kBT / kT);
+    for (int64_t i = 0; i < p->N * dims; i++) v[i] *= factor;
+}
+
+/* ------------------------------------------------------------ time steps */
+
+static int rebuild(const or_params *p, int use_cells, const double *r, const double *D,
+                   int64_t *nfull, int64_t *full, int64_t *nhalf, int64_t *half,
+                   double *disp, double *disp2, int64_t *rebuilds)
+{
+    (*rebuilds)++;                                      /* NeighborLists.jl:25 */
+    if (use_cells)
+        return or_build_lists_cells(p, r, D, nfull, full, nhalf, half, disp, disp2);
+    return or_build_lists_brute(p, r, D, nfull, full, nhalf, half, disp, disp2);
+}
+
+/* Drift with displacement bookkeeping and floor wrap (src/MD.jl:515-522,
+ * and the intended Brownian update :457-466). Returns max(disp2). */
+static double drift(const or_params *p, double *r, double *disp, double *disp2,
+                    const double *src, double scale)
+{
+    int dims = p->dims;
+    double L = p->box_size;
+    double mx = 0.0;
+    for (int64_t i = 0; i < p->N; i++) {
+        double s = 0.0;
+        for (int c = 0; c < dims; c++) {
+            double dr = scale * src[i * dims + c];
+            disp[i * dims + c] += dr;
+            double sq = disp[i * dims + c] * disp[i * dims + c];
+            s = (c == 0) ? sq : s + sq;
+            double rn = r[i * dims + c] + dr;
+            rn -= floor(rn / L) * L;
+            r[i * dims + c] = rn;
+        }
+        disp2[i] = s;
+        if (s > mx || i == 0) mx = s;
+    }
+    return mx;
+}
+
+/* Runs `nsteps` calls of do_time_step.
+ *   Newtonian/Langevin: src/MD.jl:499-554 (velocity Verlet, COM removal,
+ *   optional rescale).  Brownian: the intended semantics of src/MD.jl:429-471
+ *   (forces incl. noise, subtract mean force, r += (dt/gamma) F, wrap,
+ *   skin trigger), which the reference cannot execute as written.
+ * `noise` holds nsteps consecutive N x dims blocks (NULL for Newtonian).
+ * `use_cells` selects the cell-list rebuild (same pair set). */
+int or_do_time_steps(const or_params *p, int64_t nsteps, int use_cells,
+                     double *r, double *v, double *F, const double *D,
+                     double *disp, double *disp2,
+                     int64_t *nfull, int64_t *full, int64_t *nhalf, int64_t *half,
+                     const double *noise, int64_t *rebuilds)
+{
+    int64_t N = p->N;
+    int dims = p->dims;
+    int64_t nd = N * dims;
+    double dt = p->dt, m = p->m;
+    double thr = (p->skin_distance / 2.0) * (p->skin_distance / 2.0);
+    int rc;
+    for (int64_t s = 0; s < nsteps; s++) {
+        const double *xi = noise ? noise + s * nd : NULL;
+        if (p->system == OR_SYS_BROWNIAN) {
+            or_calculate_new_forces(p, r, D, nhalf, half, v, xi, F);
+            double Fm[3] = {0.0, 0.0, 0.0};
+            for (int64_t i = 0; i < N; i++)
+                for (int c = 0; c < dims; c++) Fm[c] += F[i * dims + c];
+            for (int c = 0; c < dims; c++) Fm[c] /= (double)N;
+            for (int64_t i = 0; i < N; i++)
+                for (int c = 0; c < dims; c++) F[i * dims + c] -= Fm[c];
+            double mx = drift(p, r, disp, disp2, F, dt / p->gamma);
+            if (mx > thr) {
+                rc = rebuild(p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+                if (rc) return rc;
+            }
+            continue;
+        }
+        for (int64_t i = 0; i < nd; i++)                  /* MD.jl:511-513 */
+            v[i] = v[i] + dt * F[i] / (2.0 * m);
+        double mx = drift(p, r, disp, disp2, v, dt);      /* MD.jl:515-522 */
+        if (mx > thr) {                                   /* MD.jl:523-525 */
+            rc = rebuild(p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+            if (rc) return rc;
+        }
+        or_calculate_new_forces(p, r, D, nhalf, half, v, xi, F);   /* MD.jl:526 */
+        for (int64_t i = 0; i < nd; i++)                  /* MD.jl:529-531 */
+            v[i] = v[i] + dt * F[i] / (2.0 * m);
+        double vm[3] = {0.0, 0.0, 0.0};                   /* MD.jl:533-541 */
+        for (int64_t i = 0; i < N; i++)
+            for (int c = 0; c < dims; c++) vm[c] += v[i * dims + c];
+        for (int c = 0; c < dims; c++) vm[c] /= (double)N;
+        for (int64_t i = 0; i < N; i++)
+            for (int c = 0; c < dims; c++) v[i * dims + c] -= vm[c];
+        if (p->rescale_temperature && p->system == OR_SYS_NEWTONIAN)   /* :542-553 */
+            or_rescale_velocities(p, v);
+    }
+    return OR_OK;
+}
+
+/* Forced rebuild: update_neighbor_lists! (src/NeighborLists.jl:24-53). */
+int or_update_neighbor_lists(const or_params *p, int use_cells, const double *r, const double *D,
+                             int64_t *nfull, int64_t *full, int64_t *nhalf, int64_t *half,
+                             double *disp, double *disp2, int64_t *rebuilds)
+{
+    return rebuild(p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+}

@@ -1,0 +1,270 @@
+"""ctypes binding of libsc (include/sc.h).
+
+The library is built in-tree (``simulationcode_amd/_lib/libsc.so``) by
+``simulationcode_amd.build``.  There is no fallback: if the library is
+missing or fails to load, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsc.so")
+
+POT_LJ = 1
+POT_BERTHIER = 2
+SYS_NEWTONIAN = 0
+SYS_LANGEVIN = 1
+SYS_BROWNIAN = 2
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+SC_OK = 0
+SC_ERR_ARG = -1
+SC_ERR_OOM = -2
+SC_ERR_OVERFLOW = -3
+SC_ERR_NONFINITE = -4
+SC_ERR_HIP = -5
+SC_ERR_STATE = -6
+
+_ERRNAMES = {
+    SC_ERR_ARG: "SC_ERR_ARG",
+    SC_ERR_OOM: "SC_ERR_OOM",
+    SC_ERR_OVERFLOW: "SC_ERR_OVERFLOW",
+    SC_ERR_NONFINITE: "SC_ERR_NONFINITE",
+    SC_ERR_HIP: "SC_ERR_HIP",
+    SC_ERR_STATE: "SC_ERR_STATE",
+}
+
+# Every symbol include/sc.h declares (checked by the CPU tests).
+EXPORTS = (
+    "sc_create", "sc_destroy", "sc_last_error", "sc_set_state", "sc_get_state",
+    "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_calculate_new_forces",
+    "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
+    "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_philox4x32_10",
+    "sc_last_elapsed_ms",
+)
+
+
+class ScError(RuntimeError):
+    """A libsc call returned an error status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class NeighborOverflowError(ScError, IndexError):
+    """Raised where the reference raises a BoundsError on a full neighbour row
+    (src/NeighborLists.jl:86)."""
+
+
+class ScParams(ctypes.Structure):
+    _fields_ = [
+        ("dims", ctypes.c_int32),
+        ("potential", ctypes.c_int32),
+        ("system", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("rescale_temperature", ctypes.c_int32),
+        ("cell_divisor", ctypes.c_int32),
+        ("N", ctypes.c_int64),
+        ("maxneighbours", ctypes.c_int64),
+        ("box_size", ctypes.c_double),
+        ("force_cutoff", ctypes.c_double),
+        ("force_cutoff2", ctypes.c_double),
+        ("skin_distance", ctypes.c_double),
+        ("lj_eps", ctypes.c_double),
+        ("lj_sigma", ctypes.c_double),
+        ("lj_Uc", ctypes.c_double),
+        ("b_c0", ctypes.c_double),
+        ("b_c2", ctypes.c_double),
+        ("b_c4", ctypes.c_double),
+        ("b_zeta", ctypes.c_double),
+        ("kBT", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("m", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libsc and declare the signatures.  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"libsc is not built ({path}); run simulationcode_amd.build.build() "
+            "(or __graft_entry__.build()) first -- there is no CPU fallback")
+    L = ctypes.CDLL(path)
+    vp, i64, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double
+    pd = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "sc_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.POINTER(ScParams), vp]),
+        "sc_destroy": (None, [vp]),
+        "sc_last_error": (ctypes.c_char_p, [vp]),
+        "sc_set_state": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+        "sc_get_state": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
+        "sc_update_neighbor_lists": (ctypes.c_int, [vp]),
+        "sc_get_neighbor_lists": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
+        "sc_calculate_new_forces": (ctypes.c_int, [vp, vp]),
+        "sc_do_time_steps": (ctypes.c_int, [vp, i64, vp]),
+        "sc_full_energy": (ctypes.c_int, [vp, pd]),
+        "sc_energy_no_neigh": (ctypes.c_int, [vp, pd]),
+        "sc_kinetic_energy": (ctypes.c_int, [vp, pd]),
+        "sc_rescale_velocities": (ctypes.c_int, [vp]),
+        "sc_rebuild_count": (ctypes.c_int, [vp, ctypes.POINTER(i64), i64]),
+        "sc_minimize": (ctypes.c_int, [vp, i64, dbl, dbl, pd]),
+        "sc_philox4x32_10": (None, [vp, vp, vp]),
+        "sc_last_elapsed_ms": (ctypes.c_int, [vp, pd]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not (a.flags["C_CONTIGUOUS"] and a.dtype == np.float64):
+        raise TypeError("arrays crossing the C-ABI must be C-contiguous float64")
+    return a.ctypes.data
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    load().sc_philox4x32_10(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+class Context:
+    """One device-resident system (a NeighborListStruct + device copy of the
+    ArrayStruct).  Thin RAII wrapper over sc_ctx."""
+
+    def __init__(self, params: ScParams, D):
+        self._lib = load()
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        if D.shape != (params.N,):
+            raise ValueError("D must have N entries")
+        self.params = params
+        self.N = int(params.N)
+        self.dims = int(params.dims)
+        h = ctypes.c_void_p()
+        rc = self._lib.sc_create(ctypes.byref(h), ctypes.byref(params), D.ctypes.data)
+        self._h = h
+        if rc != SC_OK:
+            msg = self._lib.sc_last_error(h).decode() if h.value else "invalid parameters"
+            self.close()
+            raise ScError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.sc_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != SC_OK:
+            msg = self._lib.sc_last_error(self._h).decode()
+            if rc == SC_ERR_OVERFLOW:
+                raise NeighborOverflowError(rc, msg)
+            raise ScError(rc, msg)
+
+    def _arr(self, a, per_particle=False):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        want = self.N if per_particle else self.N * self.dims
+        if a.size != want:
+            raise ValueError(f"array has {a.size} values, expected {want}")
+        return a
+
+    def set_state(self, r=None, v=None, F=None, D=None, disp=None):
+        r, v, F, disp = (self._arr(x) for x in (r, v, F, disp))
+        D = self._arr(D, per_particle=True)
+        self._check(self._lib.sc_set_state(self._h, _p(r), _p(v), _p(F), _p(D), _p(disp)))
+
+    def get_state(self, r=True, v=True, F=True, disp=False, disp2=False):
+        shp = (self.N, self.dims)
+        out = {}
+        bufs = {}
+        for name, want, shape in (("r", r, shp), ("v", v, shp), ("F", F, shp), ("disp", disp, shp),
+                                  ("disp2", disp2, (self.N,))):
+            bufs[name] = np.zeros(shape) if want else None
+        self._check(self._lib.sc_get_state(self._h, _p(bufs["r"]), _p(bufs["v"]), _p(bufs["F"]),
+                                           _p(bufs["disp"]), _p(bufs["disp2"])))
+        for k, b in bufs.items():
+            if b is not None:
+                out[k] = b
+        return out
+
+    def update_neighbor_lists(self):
+        self._check(self._lib.sc_update_neighbor_lists(self._h))
+
+    def get_neighbor_lists(self, maxneigh=None):
+        """Reference layout: counts (N,), lists as (N, maxneigh) row-per-particle
+        views of the column-major maxneigh x N arrays, 1-based."""
+        M = int(maxneigh or self.params.maxneighbours)
+        nfull = np.zeros(self.N, dtype=np.int64)
+        nhalf = np.zeros(self.N, dtype=np.int64)
+        full = np.zeros((self.N, M), dtype=np.int64)
+        half = np.zeros((self.N, M), dtype=np.int64)
+        self._check(self._lib.sc_get_neighbor_lists(self._h, M, nfull.ctypes.data, full.ctypes.data,
+                                                    nhalf.ctypes.data, half.ctypes.data))
+        return nfull, full, nhalf, half
+
+    def calculate_new_forces(self, noise=None):
+        noise = self._arr(noise)
+        self._check(self._lib.sc_calculate_new_forces(self._h, _p(noise)))
+
+    def do_time_steps(self, nsteps, noise=None):
+        if noise is not None:
+            noise = np.ascontiguousarray(noise, dtype=np.float64)
+            if noise.size != nsteps * self.N * self.dims:
+                raise ValueError("noise must hold nsteps x N x dims values")
+        self._check(self._lib.sc_do_time_steps(self._h, int(nsteps), _p(noise)))
+
+    def _scalar(self, fn):
+        out = ctypes.c_double()
+        self._check(fn(self._h, ctypes.byref(out)))
+        return out.value
+
+    def full_energy(self):
+        return self._scalar(self._lib.sc_full_energy)
+
+    def energy_no_neigh(self):
+        return self._scalar(self._lib.sc_energy_no_neigh)
+
+    def kinetic_energy(self):
+        return self._scalar(self._lib.sc_kinetic_energy)
+
+    def rescale_velocities(self):
+        self._check(self._lib.sc_rescale_velocities(self._h))
+
+    def rebuild_count(self, set_to=-1):
+        out = ctypes.c_int64()
+        self._check(self._lib.sc_rebuild_count(self._h, ctypes.byref(out), int(set_to)))
+        return out.value
+
+    def minimize(self, iters, dt, max_step):
+        mf = ctypes.c_double()
+        self._check(self._lib.sc_minimize(self._h, int(iters), float(dt), float(max_step), ctypes.byref(mf)))
+        return mf.value
+
+    def last_elapsed_ms(self):
+        return self._scalar(self._lib.sc_last_elapsed_ms)

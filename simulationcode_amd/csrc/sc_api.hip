@@ -1,0 +1,516 @@
+// sc_api.hip -- C-ABI entry points of libsc (declared in include/sc.h).
+//
+// A context owns one replica's device state and one HIP stream.  Host
+// arrays cross in the reference's AoS layout and original particle order;
+// the device keeps SoA state in cell-sorted slot order and maps through the
+// per-slot original index.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sc.h"
+#include "sc_internal.h"
+
+namespace sc {
+void philox_host(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]);
+}
+
+struct sc_ctx {
+    sc_params p{};
+    Rep h{};
+    Rep *d_rep = nullptr;
+    Tile *d_tiles = nullptr;
+    int ntiles = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    double *d_noise = nullptr;
+    size_t noise_cap = 0;
+    long long host_step = 0;
+    double last_ms = 0.0;
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int fail(sc_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                    \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, SC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define KCHK(ctx, expr)                                                            \
+    do {                                                                          \
+        if ((expr) != 0)                                                          \
+            return fail(ctx, SC_ERR_HIP, std::string("kernel launch failed: ") + \
+                                             hipGetErrorString(hipGetLastError())); \
+    } while (0)
+
+template <typename T>
+int dalloc(sc_ctx *c, T **ptr, size_t count)
+{
+    void *q = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) {
+        *ptr = nullptr;
+        return fail(c, SC_ERR_OOM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+    }
+    c->allocs.push_back(q);
+    *ptr = static_cast<T *>(q);
+    return SC_OK;
+}
+
+int push_rep(sc_ctx *c)
+{
+    HIPCHK(c, hipMemcpyAsync(c->d_rep, &c->h, sizeof(Rep), hipMemcpyHostToDevice, c->s));
+    return SC_OK;
+}
+
+// Reads the device error bits and turns them into a status.
+int check_dyn(sc_ctx *c)
+{
+    RepDyn d;
+    HIPCHK(c, hipMemcpyAsync(&d, c->h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    if (d.err & SC_EBIT_OVERFLOW)
+        return fail(c, SC_ERR_OVERFLOW,
+                    "neighbour list overflow: a particle has more than maxneighbours=" +
+                        std::to_string(c->h.cap) + " neighbours within the Verlet radius");
+    if (d.err & SC_EBIT_NONFINITE) return fail(c, SC_ERR_NONFINITE, "non-finite particle position");
+    return SC_OK;
+}
+
+int read_dyn(sc_ctx *c, RepDyn *d)
+{
+    HIPCHK(c, hipMemcpyAsync(d, c->h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
+int upload_field(sc_ctx *c, const double *src, int field)
+{
+    size_t cnt = (field == FLD_D) ? (size_t)c->h.n : (size_t)c->h.n * c->h.dims;
+    HIPCHK(c, hipMemcpyAsync(c->h.stage, src, cnt * sizeof(double), hipMemcpyHostToDevice, c->s));
+    KCHK(c, sc::launch_from_host_order(c->d_tiles, c->ntiles, c->d_rep, field, c->s));
+    return SC_OK;
+}
+
+int download_field(sc_ctx *c, double *dst, int field)
+{
+    size_t cnt = (field == FLD_D || field == FLD_DISP2) ? (size_t)c->h.n : (size_t)c->h.n * c->h.dims;
+    KCHK(c, sc::launch_to_host_order(c->d_tiles, c->ntiles, c->d_rep, field, c->s));
+    HIPCHK(c, hipMemcpyAsync(dst, c->h.stage, cnt * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
+int set_noise(sc_ctx *c, const double *noise, int64_t blocks)
+{
+    if (!noise) {
+        if (c->h.noise != nullptr) {
+            c->h.noise = nullptr;
+            return push_rep(c);
+        }
+        return SC_OK;
+    }
+    size_t need = (size_t)blocks * c->h.n * c->h.dims;
+    if (need > c->noise_cap) {
+        if (c->d_noise) {
+            HIPCHK(c, hipStreamSynchronize(c->s));
+            (void)hipFree(c->d_noise);
+            c->d_noise = nullptr;
+        }
+        void *q = nullptr;
+        if (hipMalloc(&q, need * sizeof(double)) != hipSuccess)
+            return fail(c, SC_ERR_OOM, "noise buffer allocation failed");
+        c->d_noise = static_cast<double *>(q);
+        c->noise_cap = need;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_noise, noise, need * sizeof(double), hipMemcpyHostToDevice, c->s));
+    c->h.noise = c->d_noise;
+    c->h.noise_step0 = c->host_step;
+    return push_rep(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_create(sc_ctx **out, const sc_params *p, const double *D)
+{
+    if (!out || !p || !D) return SC_ERR_ARG;
+    *out = nullptr;
+    if ((p->dims != 2 && p->dims != 3) || (p->potential != SC_POT_LJ && p->potential != SC_POT_BERTHIER) ||
+        p->system < 0 || p->system > 2 || (p->precision != SC_PREC_FP64 && p->precision != SC_PREC_MIXED) ||
+        p->N < 2 || p->N > (int64_t)1 << 30 || p->maxneighbours < 1 || p->maxneighbours > 100000 ||
+        !(p->box_size > 0.0) || !(p->skin_distance >= 0.0) || !(p->force_cutoff > 0.0))
+        return SC_ERR_ARG;
+    sc_ctx *c = new sc_ctx();
+    c->p = *p;
+    Rep &h = c->h;
+    const int n = (int)p->N;
+    double maxD = D[0];
+    for (int i = 1; i < n; i++) maxD = std::max(maxD, D[i]);
+    // initialize_neighbor_struct, src/DataStructs.jl:351-352
+    const double Rv = p->skin_distance + p->force_cutoff * maxD;
+    if (!(p->box_size > 2.0 * Rv)) {
+        int rc = fail(c, SC_ERR_ARG, "box_size must exceed twice the Verlet radius");
+        *out = c;
+        return rc;
+    }
+    h.dims = p->dims;
+    h.pot = p->potential;
+    h.sys = p->system;
+    h.prec = p->precision;
+    h.n = n;
+    h.cap = (int)p->maxneighbours;
+    h.rescale = (p->system == SC_SYS_NEWTONIAN && p->rescale_temperature) ? 1 : 0;
+    h.L = p->box_size;
+    h.invL = 1.0 / p->box_size;
+    h.fc2 = p->force_cutoff2;
+    h.rcut = std::sqrt(p->force_cutoff2);  // src/NeighborLists.jl:44
+    h.skin = p->skin_distance;
+    h.Rv2 = Rv * Rv;
+    h.thr = (p->skin_distance / 2.0) * (p->skin_distance / 2.0);
+    h.eps = p->lj_eps;
+    h.lj_s2 = p->lj_sigma * p->lj_sigma;
+    h.lj_md = p->lj_sigma * p->lj_sigma;  // find_mean_D for LJ returns sigma^2
+    h.Uc = p->lj_Uc;
+    h.c0 = p->b_c0;
+    h.c2 = p->b_c2;
+    h.c4 = p->b_c4;
+    h.zeta = p->b_zeta;
+    h.kBT = p->kBT;
+    h.gamma = p->gamma;
+    h.dt = p->dt;
+    h.m = p->m;
+    h.seed = p->seed;
+    // cells of side >= R_v / k with a (2k+1)^dims stencil; a hair of slack so
+    // floating-point cell assignment can never drop a pair at exactly R_v
+    const int k = p->cell_divisor > 0 ? p->cell_divisor : 1;
+    long long nc = (long long)std::floor(k * p->box_size / (Rv * (1.0 + 1e-10)));
+    int srange = k;
+    if (nc < 2 * k + 1) {
+        nc = 1;
+        srange = 0;
+    }
+    long long ncell = nc * nc * (p->dims == 3 ? nc : 1);
+    if (ncell > (1LL << 27)) {
+        int rc = fail(c, SC_ERR_ARG, "cell grid too large");
+        *out = c;
+        return rc;
+    }
+    h.nc = (int)nc;
+    h.ncell = (int)ncell;
+    h.srange = srange;
+    h.cinv = (double)nc / p->box_size;
+    c->ntiles = (n + SC_TB - 1) / SC_TB;
+    h.ntiles = c->ntiles;
+    h.tile0 = 0;
+    *out = c;
+
+    if (hipStreamCreate(&c->s) != hipSuccess) return fail(c, SC_ERR_HIP, "hipStreamCreate failed");
+    HIPCHK(c, hipEventCreate(&c->ev0));
+    HIPCHK(c, hipEventCreate(&c->ev1));
+    const int d = p->dims;
+    int rc = SC_OK;
+    rc |= dalloc(c, &h.pos, n);
+    rc |= dalloc(c, &h.pos_alt, n);
+    rc |= dalloc(c, &h.v, (size_t)n * d);
+    rc |= dalloc(c, &h.v_alt, (size_t)n * d);
+    rc |= dalloc(c, &h.F, (size_t)n * d);
+    rc |= dalloc(c, &h.disp, (size_t)n * d);
+    rc |= dalloc(c, &h.orig, n);
+    rc |= dalloc(c, &h.orig_alt, n);
+    rc |= dalloc(c, &h.cell_of, n);
+    rc |= dalloc(c, &h.rank, n);
+    rc |= dalloc(c, &h.perm, n);
+    rc |= dalloc(c, &h.cell_cnt, (size_t)ncell);
+    rc |= dalloc(c, &h.cell_start, (size_t)ncell + 1);
+    rc |= dalloc(c, &h.nn, n);
+    rc |= dalloc(c, &h.nbr, (size_t)n * h.cap);
+    rc |= dalloc(c, &h.part, (size_t)c->ntiles * 8);
+    rc |= dalloc(c, &h.stage, (size_t)n * 3);
+    rc |= dalloc(c, &h.dyn, 1);
+    rc |= dalloc(c, &c->d_rep, 1);
+    rc |= dalloc(c, &c->d_tiles, c->ntiles);
+    if (rc != SC_OK) return SC_ERR_OOM;
+
+    std::vector<Tile> tiles(c->ntiles);
+    for (int t = 0; t < c->ntiles; t++) tiles[t] = Tile{0, t * SC_TB, t, 0};
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; i++) ident[i] = i;
+    std::vector<double4> pos(n);
+    for (int i = 0; i < n; i++) pos[i] = make_double4(0.0, 0.0, 0.0, D[i]);
+    RepDyn dyn;
+    std::memset(&dyn, 0, sizeof(dyn));
+    dyn.vscale = 1.0;
+    HIPCHK(c, hipMemcpy(c->d_tiles, tiles.data(), sizeof(Tile) * c->ntiles, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(h.orig, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(h.pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemset(h.v, 0, sizeof(double) * n * d));
+    HIPCHK(c, hipMemset(h.F, 0, sizeof(double) * n * d));
+    HIPCHK(c, hipMemset(h.disp, 0, sizeof(double) * n * d));
+    HIPCHK(c, hipMemset(h.cell_cnt, 0, sizeof(int) * ncell));
+    HIPCHK(c, hipMemset(h.nn, 0, sizeof(int) * n));
+    HIPCHK(c, hipMemset(h.part, 0, sizeof(double) * c->ntiles * 8));
+    HIPCHK(c, hipMemcpy(h.dyn, &dyn, sizeof(RepDyn), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_rep, &h, sizeof(Rep), hipMemcpyHostToDevice));
+    return SC_OK;
+}
+
+void sc_destroy(sc_ctx *c)
+{
+    if (!c) return;
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    for (void *q : c->allocs) (void)hipFree(q);
+    if (c->d_noise) (void)hipFree(c->d_noise);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+}
+
+const char *sc_last_error(const sc_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, const double *D,
+                 const double *disp)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    int rc;
+    if (r && (rc = upload_field(c, r, FLD_R))) return rc;
+    if (v && (rc = upload_field(c, v, FLD_V))) return rc;
+    if (F && (rc = upload_field(c, F, FLD_F))) return rc;
+    if (D && (rc = upload_field(c, D, FLD_D))) return rc;
+    if (disp && (rc = upload_field(c, disp, FLD_DISP))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
+int sc_get_state(sc_ctx *c, double *r, double *v, double *F, double *disp, double *disp2)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    int rc;
+    if (r && (rc = download_field(c, r, FLD_R))) return rc;
+    if (v && (rc = download_field(c, v, FLD_V))) return rc;
+    if (F && (rc = download_field(c, F, FLD_F))) return rc;
+    if (disp && (rc = download_field(c, disp, FLD_DISP))) return rc;
+    if (disp2 && (rc = download_field(c, disp2, FLD_DISP2))) return rc;
+    return SC_OK;
+}
+
+int sc_update_neighbor_lists(sc_ctx *c)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    KCHK(c, sc::launch_request_rebuild(c->d_rep, 1, c->s));
+    KCHK(c, sc::launch_rebuild(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
+    return check_dyn(c);
+}
+
+int sc_get_neighbor_lists(sc_ctx *c, int64_t maxneigh, int64_t *nfull, int64_t *full, int64_t *nhalf,
+                          int64_t *half)
+{
+    if (!c || !c->d_rep || maxneigh < 1) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    const int n = c->h.n, cap = c->h.cap;
+    if (nfull || full) {
+        int *rows = nullptr, *cnts = nullptr;
+        if (hipMalloc(&rows, sizeof(int) * (size_t)n * cap) != hipSuccess ||
+            hipMalloc(&cnts, sizeof(int) * (size_t)n) != hipSuccess) {
+            (void)hipFree(rows);
+            return fail(c, SC_ERR_OOM, "list export buffers");
+        }
+        std::vector<int> hr((size_t)n * cap), hc(n);
+        KCHK(c, sc::launch_full_rows(c->d_tiles, c->ntiles, c->d_rep, rows, cnts, cap, c->s));
+        HIPCHK(c, hipMemcpyAsync(hr.data(), rows, sizeof(int) * hr.size(), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipMemcpyAsync(hc.data(), cnts, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        (void)hipFree(rows);
+        (void)hipFree(cnts);
+        for (int i = 0; i < n; i++) {
+            if (hc[i] > cap || hc[i] > maxneigh)
+                return fail(c, SC_ERR_OVERFLOW, "full list row longer than maxneigh");
+            if (nfull) nfull[i] = hc[i];
+            if (full)
+                for (int k = 0; k < hc[i]; k++) full[(size_t)i * maxneigh + k] = (int64_t)hr[(size_t)i * cap + k] + 1;
+        }
+    }
+    if (nhalf || half) {
+        std::vector<int> nn(n), orig(n), nb((size_t)n * cap);
+        HIPCHK(c, hipMemcpyAsync(nn.data(), c->h.nn, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipMemcpyAsync(orig.data(), c->h.orig, sizeof(int) * n, hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipMemcpyAsync(nb.data(), c->h.nbr, sizeof(int) * nb.size(), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        std::vector<int64_t> cnt(n, 0);
+        for (int k = 0; k < n; k++) {
+            const int a = orig[k];
+            for (int q = 0; q < nn[k]; q++) {
+                const int b = orig[nb[(size_t)q * n + k]];
+                if (a < b) {
+                    if (cnt[a] >= maxneigh) return fail(c, SC_ERR_OVERFLOW, "half list row longer than maxneigh");
+                    if (half) half[(size_t)a * maxneigh + cnt[a]] = (int64_t)b + 1;
+                    cnt[a]++;
+                }
+            }
+        }
+        if (nhalf)
+            for (int i = 0; i < n; i++) nhalf[i] = cnt[i];
+    }
+    return SC_OK;
+}
+
+int sc_calculate_new_forces(sc_ctx *c, const double *noise)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, 1))) return rc;
+    KCHK(c, sc::launch_force(c->d_tiles, c->ntiles, c->d_rep, 0, 7, c->s));
+    c->host_step += 1;
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
+int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
+{
+    if (!c || !c->d_rep || nsteps < 0) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, nsteps))) return rc;
+    const int nt = c->ntiles;
+    HIPCHK(c, hipEventRecord(c->ev0, c->s));
+    for (int64_t s = 0; s < nsteps; s++) {
+        if (c->h.sys == SC_SYS_BROWNIAN) {
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 2, 7, c->s));
+            KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, c->s));
+            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+        } else {
+            KCHK(c, sc::launch_kick_drift(c->d_tiles, nt, c->d_rep, c->s));
+            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 7, c->s));
+            if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, c->s));
+        }
+    }
+    KCHK(c, sc::launch_finalize(c->d_tiles, nt, c->d_rep, c->s));
+    HIPCHK(c, hipEventRecord(c->ev1, c->s));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->host_step += nsteps;
+    return check_dyn(c);
+}
+
+int sc_full_energy(sc_ctx *c, double *out)
+{
+    if (!c || !c->d_rep || !out) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    KCHK(c, sc::launch_energy(c->d_tiles, c->ntiles, c->d_rep, c->s));
+    if ((rc = read_dyn(c, &d))) return rc;
+    *out = d.result[0];
+    return SC_OK;
+}
+
+int sc_energy_no_neigh(sc_ctx *c, double *out)
+{
+    if (!c || !c->d_rep || !out) return SC_ERR_ARG;
+    KCHK(c, sc::launch_energy_nn(c->d_tiles, c->ntiles, c->d_rep, c->s));
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    *out = d.result[0];
+    return SC_OK;
+}
+
+int sc_kinetic_energy(sc_ctx *c, double *out)
+{
+    if (!c || !c->d_rep || !out) return SC_ERR_ARG;
+    KCHK(c, sc::launch_ekin(c->d_tiles, c->ntiles, c->d_rep, 0, c->s));
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    *out = d.result[1];
+    return SC_OK;
+}
+
+int sc_rescale_velocities(sc_ctx *c)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    KCHK(c, sc::launch_ekin(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
+    KCHK(c, sc::launch_finalize(c->d_tiles, c->ntiles, c->d_rep, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
+int sc_rebuild_count(sc_ctx *c, int64_t *out, int64_t set)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (out) *out = d.rebuilds;
+    if (set >= 0) {
+        long long v = set;
+        HIPCHK(c, hipMemcpyAsync(reinterpret_cast<char *>(c->h.dyn) + offsetof(RepDyn, rebuilds), &v, sizeof(v),
+                                 hipMemcpyHostToDevice, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+    }
+    return SC_OK;
+}
+
+int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *maxforce)
+{
+    if (!c || !c->d_rep || iters < 0 || !(dt > 0.0) || !(max_step > 0.0)) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    const int nt = c->ntiles;
+    // pair forces only: a Newtonian view of the replica for the force kernel
+    Rep saved = c->h;
+    c->h.sys = SC_SYS_NEWTONIAN;
+    if ((rc = push_rep(c))) return rc;
+    for (int64_t it = 0; it < iters; it++) {
+        KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 0, 7, c->s));
+        KCHK(c, sc::launch_min_scale(c->d_tiles, nt, c->d_rep, dt, max_step, c->s));
+        KCHK(c, sc::launch_min_move(c->d_tiles, nt, c->d_rep, dt, c->s));
+        KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+    }
+    c->host_step += iters;
+    c->h = saved;
+    if ((rc = push_rep(c))) return rc;
+    if ((rc = read_dyn(c, &d))) return rc;
+    if (maxforce) *maxforce = d.result[3];
+    return check_dyn(c);
+}
+
+int sc_last_elapsed_ms(sc_ctx *c, double *ms)
+{
+    if (!c || !ms) return SC_ERR_ARG;
+    *ms = c->last_ms;
+    return SC_OK;
+}
+
+void sc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { sc::philox_host(ctr, key, out); }
+
+}  // extern "C"

@@ -1,0 +1,103 @@
+// sc_internal.h -- device-side data layout of libsc (not part of the C-ABI).
+//
+// One "replica" is one independent periodic system (the reference's
+// ArrayStruct + NeighborListStruct).  Every kernel runs over a table of
+// tiles; a tile is SC_TB consecutive particle slots of one replica, so one
+// launch can serve one system or many replicas of uneven N and mixed 2D/3D.
+//
+// HBM layout per replica (n particles, slots sorted by cell at each rebuild):
+//   pos    double4[n]   {x, y, z, D}   (z = 0 in 2D); D rides with r because
+//                                      every pair evaluation needs both
+//   v, F, disp          SoA doubles, component c at [c*n + i]
+//   orig   int[n]       original (host) index of the particle in slot i
+//   nn     int[n]       short-list length of slot i
+//   nbr    int[cap*n]   short list, entry k of slot i at [k*n + i] so that a
+//                       wavefront of consecutive slots reads consecutive words
+//   cell_start int[ncell+1]  slot range of each cell (cell-sorted order)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SC_TB 256  // threads per tile / block (4 wavefronts)
+
+// error bits raised on the device
+#define SC_EBIT_OVERFLOW 1
+#define SC_EBIT_NONFINITE 2
+
+// last-block-done counters
+enum { CTR_FORCE = 0, CTR_DRIFT, CTR_BUILD, CTR_ENERGY, CTR_EKIN, CTR_MISC, CTR_EKCOM, CTR_N };
+
+struct RepDyn {
+    unsigned long long maxdisp_bits;  // atomicMax over disp^2 (non-negative doubles)
+    unsigned long long maxaux_bits;   // atomicMax scratch (minimiser step / force)
+    int rebuild;                      // 1 => the rebuild kernels run
+    int err;                          // SC_EBIT_*
+    int pending;                      // 1 => vmean / vscale not yet applied to v
+    int have_lists;
+    unsigned int ctr[CTR_N];
+    long long rebuilds;
+    long long step;                   // force evaluations so far (noise counter)
+    double vmean[3];
+    double vscale;
+    double Fmean[3];
+    double result[4];                 // [0] energy, [1] kinetic, [2] minimiser scale, [3] max |F|
+};
+
+struct Rep {
+    // ---- geometry / physics (immutable after creation)
+    int dims, pot, sys, prec;
+    int n;               // particles
+    int nc;              // cells per side
+    int ncell;           // nc^dims
+    int srange;          // stencil half-width in cells (0 => single cell, all pairs)
+    int cap;             // maxneighbours (row capacity)
+    int rescale;
+    int ntiles, tile0;   // tiles of this replica in the launch table
+    double L, invL, cinv;          // box, 1/box, cells per unit length
+    double fc2, rcut, skin, Rv2, thr;
+    double eps, lj_s2, lj_md, Uc;  // LJ: sigma^2 (specialised loops), find_mean_D = sigma^2
+    double c0, c2, c4, zeta;
+    double kBT, gamma, dt, m;
+    uint64_t seed;
+    // ---- injected noise (NULL => Philox): block s at noise + (step - noise_step0) * n * dims
+    const double *noise;
+    long long noise_step0;
+    // ---- device buffers
+    double4 *pos, *pos_alt;
+    double *v, *v_alt, *F, *disp;
+    int *orig, *orig_alt;
+    int *cell_of, *rank, *cell_cnt, *cell_start, *perm;
+    int *nn, *nbr;
+    double *part;        // per-tile partial sums, 8 doubles per tile
+    double *stage;       // n*3 doubles staging for host transfers
+    RepDyn *dyn;
+};
+
+struct Tile {
+    int rep;
+    int first;  // first slot of the tile within its replica
+    int local;  // tile index within the replica
+    int pad;
+};
+
+// launchers (sc_kernels.hip)
+namespace sc {
+int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int mode, int sysmask, hipStream_t s);
+int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
+int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int apply_rescale, hipStream_t s);
+int launch_from_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
+int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
+int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, int *counts, int cap, hipStream_t s);
+int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, double dt, double max_step, hipStream_t s);
+int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, double dt, hipStream_t s);
+}  // namespace sc
+
+// fields for the host-order transfer kernels
+enum { FLD_R = 0, FLD_V, FLD_F, FLD_D, FLD_DISP, FLD_DISP2 };

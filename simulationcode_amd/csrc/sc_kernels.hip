@@ -1,0 +1,1311 @@
+// sc_kernels.hip -- gfx950 kernels of libsc.
+//
+// Compiled with -ffp-contract=off: every FP64 expression rounds exactly as
+// written, in the operation order of the reference loops it restates, so that
+// squared distances (and therefore neighbour-pair sets) are bit-identical to
+// the reference's calculate_squared_distance (src/AuxiliaryFunctions.jl:311-315).
+//
+// Execution model: one thread per particle slot, SC_TB (=256) slots per
+// tile, one block per tile (grid = tiles of all replicas in the launch).
+// Per-replica reductions (COM velocity, mean force, max displacement,
+// energies) are deterministic: tile partials in a fixed order, finished by
+// the last tile to arrive (atomic counter), which also resets the counter so
+// the launch sequence can be replayed (hipGraph) without host work.
+#include "sc_internal.h"
+
+#include <math.h>
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int NWAVES = SC_TB / WAVE;
+
+// ------------------------------------------------------------ reductions
+
+__device__ inline double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, WAVE);
+    return x;
+}
+
+__device__ inline double wave_max(double x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, WAVE));
+    return x;
+}
+
+// Block sum of K values per thread; result valid in all threads.
+template <int K>
+__device__ inline void block_sum(double (&x)[K])
+{
+    __shared__ double red[NWAVES][K > 0 ? K : 1];
+    int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < K; k++) x[k] = wave_sum(x[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) red[w][k] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = red[0][k];
+#pragma unroll
+        for (int q = 1; q < NWAVES; q++) s += red[q][k];
+        x[k] = s;
+    }
+    __syncthreads();
+}
+
+__device__ inline double block_max(double x)
+{
+    __shared__ double red[NWAVES];
+    int lane = threadIdx.x % WAVE, w = threadIdx.x / WAVE;
+    x = wave_max(x);
+    __syncthreads();
+    if (lane == 0) red[w] = x;
+    __syncthreads();
+    double m = red[0];
+#pragma unroll
+    for (int q = 1; q < NWAVES; q++) m = fmax(m, red[q]);
+    __syncthreads();
+    return m;
+}
+
+// Neumaier-compensated pair (sum, compensation) helpers.
+__device__ inline void neumaier_add(double &s, double &c, double x)
+{
+    double t = s + x;
+    if (fabs(s) >= fabs(x))
+        c += (s - t) + x;
+    else
+        c += (x - t) + s;
+    s = t;
+}
+
+// Last-tile-done: returns true in every thread of the tile that arrives last
+// for this replica's counter k; that tile must reset the counter.
+__device__ inline bool arrive_last(RepDyn *dyn, int k, int ntiles)
+{
+    __shared__ int last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned prev = atomicAdd(&dyn->ctr[k], 1u);
+        last = (prev == (unsigned)(ntiles - 1));
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last != 0;
+}
+
+// Sum the per-tile partials (K values each, stride 8) in a fixed order.
+template <int K>
+__device__ inline void reduce_partials(const Rep &R, double (&out)[K])
+{
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) acc[k] = 0.0;
+    for (int t = threadIdx.x; t < R.ntiles; t += SC_TB)
+#pragma unroll
+        for (int k = 0; k < K; k++) acc[k] += __builtin_nontemporal_load(&R.part[t * 8 + k]);
+    block_sum<K>(acc);
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = acc[k];
+}
+
+__device__ inline void atomic_max_nonneg(unsigned long long *p, double x)
+{
+    atomicMax(p, (unsigned long long)__double_as_longlong(x));
+}
+
+// ------------------------------------------------------------ Philox4x32-10
+// Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11.  Ten rounds, key bumped between rounds.
+__host__ __device__ inline void philox4x32_10(const uint32_t in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t x0 = in[0], x1 = in[1], x2 = in[2], x3 = in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        if (r > 0) { k0 += W0; k1 += W1; }
+        uint64_t p0 = (uint64_t)M0 * x0, p1 = (uint64_t)M1 * x2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t y0 = hi1 ^ x1 ^ k0, y2 = hi0 ^ x3 ^ k1;
+        x0 = y0; x1 = lo1; x2 = y2; x3 = lo0;
+    }
+    out[0] = x0; out[1] = x1; out[2] = x2; out[3] = x3;
+}
+
+// Two standard normals (Box-Muller) from one Philox block.
+__device__ inline void normals2(uint32_t orig, long long step, uint32_t sub, uint64_t seed, double &z0, double &z1)
+{
+    uint32_t c[4] = {orig, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32), sub};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    philox4x32_10(c, k, o);
+    uint64_t a = ((uint64_t)o[0] << 32) | o[1];
+    uint64_t b = ((uint64_t)o[2] << 32) | o[3];
+    double u1 = ((double)(a >> 11) + 1.0) * 0x1.0p-53;  // (0, 1]
+    double u2 = (double)(b >> 11) * 0x1.0p-53;          // [0, 1)
+    double rr = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    z0 = rr * cs;
+    z1 = rr * sn;
+}
+
+// --------------------------------------------------------- pair physics
+
+// Minimum-image component: dx - round(dx/L)*L (src/AuxiliaryFunctions.jl:313).
+// rint(dx*invL) equals rint(dx/L) except when dx/L is within an ulp of a
+// half-integer, i.e. |dx| ~ L/2, where the wrapped distance is ~L/2 either
+// way -- beyond every list radius and cutoff (L > 2 R_v is enforced), so no
+// predicate or force can change.
+__device__ inline double min_image(double dx, double L, double invL)
+{
+    double n = rint(dx * invL);
+    double t = n * L;
+    return dx - t;
+}
+
+// FP64 force/r of one pair, in the operation order of the reference loops:
+// Berthier src/MD.jl:120-126 (3D) / :228-234 (2D); LJ 3D :175-180; generic
+// (2D LJ) :299-307 with force() of src/InteractionPotential.jl:145-152.
+template <int DIMS, int POT>
+__device__ inline double pair_fdivr64(const Rep &R, double rij2, double Di, double Dj)
+{
+    if (POT == 2) {
+        double md = (Di + Dj) * 0.5 * (1.0 - R.zeta * fabs(Di - Dj));
+        double md2 = md * md;
+        double inv_md2 = 1.0 / md2;
+        double invxi2 = rij2 * inv_md2;
+        double invxi4 = invxi2 * invxi2;
+        double xi14 = 1.0 / (invxi4 * invxi4 * invxi4 * invxi2);
+        return (R.fc2 * md2 < rij2) ? 0.0 : -2.0 * inv_md2 * (R.c2 + 2.0 * R.c4 * invxi2 - 6.0 * xi14);
+    } else if (DIMS == 3) {
+        double xi2 = R.lj_s2 / rij2;
+        double xi6 = xi2 * xi2 * xi2;
+        double xi12 = xi6 * xi6;
+        return (R.fc2 < rij2) ? 0.0 : R.eps * (48.0 * xi12 - 24.0 * xi6) / rij2;
+    } else {
+        double md2 = R.lj_md * R.lj_md;
+        if (R.fc2 * md2 > rij2) {
+            double inv_r2 = 1.0 / rij2;
+            double xi2 = md2 * inv_r2;
+            double xi6 = xi2 * xi2 * xi2;
+            double xi12 = xi6 * xi6;
+            return R.eps * (48.0 * xi12 - 24.0 * xi6) * inv_r2;
+        }
+        return 0.0;
+    }
+}
+
+// FP32 force/r (mixed precision): same formulas, FP32 arithmetic with
+// hardware reciprocals.
+template <int DIMS, int POT>
+__device__ inline float pair_fdivr32(const Rep &R, float rij2, float Di, float Dj)
+{
+    if (POT == 2) {
+        float md = (Di + Dj) * 0.5f * (1.0f - (float)R.zeta * fabsf(Di - Dj));
+        float md2 = md * md;
+        float inv_md2 = __builtin_amdgcn_rcpf(md2);
+        float invxi2 = rij2 * inv_md2;
+        float invxi4 = invxi2 * invxi2;
+        float xi14 = __builtin_amdgcn_rcpf(invxi4 * invxi4 * invxi4 * invxi2);
+        return ((float)R.fc2 * md2 < rij2) ? 0.0f
+               : -2.0f * inv_md2 * ((float)R.c2 + 2.0f * (float)R.c4 * invxi2 - 6.0f * xi14);
+    } else if (DIMS == 3) {
+        float inv_r2 = __builtin_amdgcn_rcpf(rij2);
+        float xi2 = (float)R.lj_s2 * inv_r2;
+        float xi6 = xi2 * xi2 * xi2;
+        float xi12 = xi6 * xi6;
+        return ((float)R.fc2 < rij2) ? 0.0f : (float)R.eps * (48.0f * xi12 - 24.0f * xi6) * inv_r2;
+    } else {
+        float md2 = (float)(R.lj_md * R.lj_md);
+        if ((float)R.fc2 * md2 > rij2) {
+            float inv_r2 = __builtin_amdgcn_rcpf(rij2);
+            float xi2 = md2 * inv_r2;
+            float xi6 = xi2 * xi2 * xi2;
+            float xi12 = xi6 * xi6;
+            return (float)R.eps * (48.0f * xi12 - 24.0f * xi6) * inv_r2;
+        }
+        return 0.0f;
+    }
+}
+
+// Pair energy with the cutoff test of find_energy's inner loops
+// (src/AuxiliaryFunctions.jl:381-413, :416-444, :477-507; generic :568-581).
+template <int DIMS, int POT>
+__device__ inline double pair_energy(const Rep &R, double rij2, double Di, double Dj)
+{
+    if (POT == 2) {
+        double md = (Di + Dj) * 0.5 * (1.0 - R.zeta * fabs(Di - Dj));
+        double md2 = md * md;
+        double invxi2 = rij2 / md2;
+        double invxi4 = invxi2 * invxi2;
+        double xi12 = 1.0 / (invxi4 * invxi4 * invxi4);
+        double e = xi12 + R.c4 * invxi4 + R.c2 * invxi2 + R.c0;
+        return (R.fc2 * md2 < rij2) ? 0.0 : e;
+    } else if (DIMS == 3) {
+        double md2 = R.lj_s2;  // sigma * sigma
+        double xi2 = md2 / rij2;
+        double xi6 = xi2 * xi2 * xi2;
+        double xi12 = xi6 * xi6;
+        double e = 4.0 * R.eps * (xi12 - xi6) - R.Uc;
+        return (R.fc2 * md2 < rij2) ? 0.0 : e;
+    } else {
+        double md2 = R.lj_md * R.lj_md;
+        if (R.fc2 * md2 > rij2) {
+            double xi2 = md2 / rij2;
+            double xi4 = xi2 * xi2;
+            double xi6 = xi2 * xi4;
+            double xi12 = xi6 * xi6;
+            return 4.0 * R.eps * (xi12 - xi6) - R.Uc;
+        }
+        return 0.0;
+    }
+}
+
+// Potential of calculate_full_energy_no_neigh (generic potential(),
+// src/AuxiliaryFunctions.jl:94-108 with src/InteractionPotential.jl:123-131,
+// :193-201): mean_d from find_mean_D for every potential.
+template <int POT>
+__device__ inline double pair_energy_generic(const Rep &R, double rij2, double Di, double Dj)
+{
+    double md, md2;
+    if (POT == 2) {
+        md = (Di + Dj) * 0.5 * (1.0 - R.zeta * fabs(Di - Dj));
+        md2 = md * md;
+        double invxi2 = rij2 / md2;
+        double invxi4 = invxi2 * invxi2;
+        double xi12 = 1.0 / (invxi4 * invxi4 * invxi4);
+        double e = xi12 + R.c4 * invxi4 + R.c2 * invxi2 + R.c0;
+        return (R.fc2 * md2 < rij2) ? 0.0 : e;
+    }
+    md = R.lj_md;
+    md2 = md * md;
+    double xi2 = md2 / rij2;
+    double xi4 = xi2 * xi2;
+    double xi6 = xi2 * xi4;
+    double xi12 = xi6 * xi6;
+    double e = 4.0 * R.eps * (xi12 - xi6) - R.Uc;
+    return (R.fc2 * md2 < rij2) ? 0.0 : e;
+}
+
+// Squared minimum-image distance, left fold (src/AuxiliaryFunctions.jl:311-315).
+template <int DIMS>
+__device__ inline double sqdist(const Rep &R, double4 a, double4 b, double &dx, double &dy, double &dz)
+{
+    dx = min_image(a.x - b.x, R.L, R.invL);
+    dy = min_image(a.y - b.y, R.L, R.invL);
+    double d2 = dx * dx + dy * dy;
+    if (DIMS == 3) {
+        dz = min_image(a.z - b.z, R.L, R.invL);
+        d2 = d2 + dz * dz;
+    } else {
+        dz = 0.0;
+    }
+    return d2;
+}
+
+// Cell coordinate of a wrapped position component.
+__device__ inline int cell_coord(double x, const Rep &R)
+{
+    int c = (int)floor(x * R.cinv);
+    c = c < 0 ? 0 : c;
+    return c >= R.nc ? R.nc - 1 : c;
+}
+
+template <int DIMS>
+__device__ inline void cell_of_pos(const Rep &R, double4 p, int (&cc)[3])
+{
+    cc[0] = cell_coord(p.x, R);
+    cc[1] = cell_coord(p.y, R);
+    cc[2] = DIMS == 3 ? cell_coord(p.z, R) : 0;
+}
+
+__device__ inline int wrapc(int c, int nc) { return c < 0 ? c + nc : (c >= nc ? c - nc : c); }
+
+// ====================================================================
+// Integrator kernels
+// ====================================================================
+
+// First half-kick + drift + displacement bookkeeping + floor wrap + skin
+// trigger: src/MD.jl:511-525.  Applies the pending COM shift / rescale of
+// the previous step first (src/MD.jl:533-553), in the reference's order.
+template <int DIMS>
+__device__ void kick_drift_tile(const Rep &R, const Tile &t)
+{
+    RepDyn *dyn = R.dyn;
+    const int n = R.n;
+    const int i = t.first + threadIdx.x;
+    const int pend = dyn->pending;
+    double s2 = 0.0;
+    if (i < n) {
+        double4 p = R.pos[i];
+        double pc[3] = {p.x, p.y, p.z};
+        double s = 0.0;
+        bool fin = true;
+#pragma unroll
+        for (int c = 0; c < DIMS; c++) {
+            double vc = R.v[c * n + i];
+            if (pend & 1) vc = vc - dyn->vmean[c];
+            if (pend & 2) vc = vc * dyn->vscale;
+            vc = vc + R.dt * R.F[c * n + i] / (2.0 * R.m);
+            double dr = R.dt * vc;
+            double dc = R.disp[c * n + i] + dr;
+            s = (c == 0) ? dc * dc : s + dc * dc;
+            double rn = pc[c] + dr;
+            rn -= floor(rn / R.L) * R.L;
+            pc[c] = rn;
+            fin = fin && isfinite(rn);
+            R.v[c * n + i] = vc;
+            R.disp[c * n + i] = dc;
+        }
+        R.pos[i] = make_double4(pc[0], pc[1], pc[2], p.w);
+        s2 = s;
+        if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
+    }
+    double m = block_max(s2);
+    if (threadIdx.x == 0) atomic_max_nonneg(&dyn->maxdisp_bits, m);
+    if (arrive_last(dyn, CTR_DRIFT, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            double mx = __longlong_as_double((long long)atomicAdd(&dyn->maxdisp_bits, 0ull));
+            if (mx > R.thr) dyn->rebuild = 1;
+            dyn->maxdisp_bits = 0ull;
+            dyn->pending = 0;
+            dyn->ctr[CTR_DRIFT] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (R.sys == 2) return;
+    if (R.dims == 3)
+        kick_drift_tile<3>(R, t);
+    else
+        kick_drift_tile<2>(R, t);
+}
+
+// Brownian drift: F -= mean(F); r += (dt/gamma) F; bookkeeping; wrap;
+// trigger.  The intended semantics of src/MD.jl:445-470.
+template <int DIMS>
+__device__ void drift_brownian_tile(const Rep &R, const Tile &t)
+{
+    RepDyn *dyn = R.dyn;
+    const int n = R.n;
+    const int i = t.first + threadIdx.x;
+    double s2 = 0.0;
+    if (i < n) {
+        double4 p = R.pos[i];
+        double pc[3] = {p.x, p.y, p.z};
+        double factor = R.dt / R.gamma;
+        double s = 0.0;
+        bool fin = true;
+#pragma unroll
+        for (int c = 0; c < DIMS; c++) {
+            double fc = R.F[c * n + i] - dyn->Fmean[c];
+            double dr = factor * fc;
+            double dc = R.disp[c * n + i] + dr;
+            s = (c == 0) ? dc * dc : s + dc * dc;
+            double rn = pc[c] + dr;
+            rn -= floor(rn / R.L) * R.L;
+            pc[c] = rn;
+            fin = fin && isfinite(rn);
+            R.F[c * n + i] = fc;
+            R.disp[c * n + i] = dc;
+        }
+        R.pos[i] = make_double4(pc[0], pc[1], pc[2], p.w);
+        s2 = s;
+        if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
+    }
+    double m = block_max(s2);
+    if (threadIdx.x == 0) atomic_max_nonneg(&dyn->maxdisp_bits, m);
+    if (arrive_last(dyn, CTR_DRIFT, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            double mx = __longlong_as_double((long long)atomicAdd(&dyn->maxdisp_bits, 0ull));
+            if (mx > R.thr) dyn->rebuild = 1;
+            dyn->maxdisp_bits = 0ull;
+            dyn->ctr[CTR_DRIFT] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (R.sys != 2) return;
+    if (R.dims == 3)
+        drift_brownian_tile<3>(R, t);
+    else
+        drift_brownian_tile<2>(R, t);
+}
+
+// Pair forces over the symmetric short list (gather only, no atomics, so the
+// result is bitwise repeatable), then add_new_random_force! and, in MODE 1,
+// the second half-kick with COM partial sums.
+//   MODE 0: calculate_new_forces! only (src/MD.jl:262-316)
+//   MODE 1: + v += dt F/(2m) and COM velocity (src/MD.jl:526-541)
+//   MODE 2: Brownian: + mean-force partials (src/MD.jl:442-454)
+template <int DIMS, int POT, int PREC, int MODE>
+__device__ void force_tile(const Rep &R, const Tile &t)
+{
+    RepDyn *dyn = R.dyn;
+    const int n = R.n;
+    const int i = t.first + threadIdx.x;
+    const long long step = dyn->step;
+    double f[3] = {0.0, 0.0, 0.0};
+    double red[3] = {0.0, 0.0, 0.0};
+    if (i < n) {
+        const double4 pi = R.pos[i];
+        const int cnt = R.nn[i];
+        const int *nb = R.nbr + i;
+        if (PREC == 0) {
+            for (int k = 0; k < cnt; k++) {
+                const int j = nb[(size_t)k * n];
+                const double4 pj = R.pos[j];
+                double dx, dy, dz;
+                double rij2 = sqdist<DIMS>(R, pi, pj, dx, dy, dz);
+                double fs = pair_fdivr64<DIMS, POT>(R, rij2, pi.w, pj.w);
+                f[0] += fs * dx;
+                f[1] += fs * dy;
+                if (DIMS == 3) f[2] += fs * dz;
+            }
+        } else {
+            float g[3] = {0.0f, 0.0f, 0.0f};
+            const float Di = (float)pi.w;
+            for (int k = 0; k < cnt; k++) {
+                const int j = nb[(size_t)k * n];
+                const double4 pj = R.pos[j];
+                float dx = (float)min_image(pi.x - pj.x, R.L, R.invL);
+                float dy = (float)min_image(pi.y - pj.y, R.L, R.invL);
+                float dz = DIMS == 3 ? (float)min_image(pi.z - pj.z, R.L, R.invL) : 0.0f;
+                float rij2 = dx * dx + dy * dy;
+                if (DIMS == 3) rij2 = rij2 + dz * dz;
+                float fs = pair_fdivr32<DIMS, POT>(R, rij2, Di, (float)pj.w);
+                g[0] += fs * dx;
+                g[1] += fs * dy;
+                if (DIMS == 3) g[2] += fs * dz;
+            }
+            f[0] = g[0];
+            f[1] = g[1];
+            f[2] = g[2];
+        }
+        // add_new_random_force! (src/MD.jl:30-43 Langevin, :67-78 Brownian)
+        if (R.sys != 0) {
+            double xi[3];
+            if (R.noise) {
+                const double *nz = R.noise + (size_t)(step - R.noise_step0) * (size_t)n * DIMS +
+                                   (size_t)R.orig[i] * DIMS;
+#pragma unroll
+                for (int c = 0; c < DIMS; c++) xi[c] = nz[c];
+            } else {
+                double z0, z1, z2, z3;
+                normals2((uint32_t)R.orig[i], step, 0u, R.seed, z0, z1);
+                xi[0] = z0;
+                xi[1] = z1;
+                if (DIMS == 3) {
+                    normals2((uint32_t)R.orig[i], step, 1u, R.seed, z2, z3);
+                    xi[2] = z2;
+                }
+            }
+            double factor = sqrt(2.0 * R.kBT * R.gamma / R.dt);
+#pragma unroll
+            for (int c = 0; c < DIMS; c++) {
+                double add = (R.sys == 1) ? -R.gamma * R.v[c * n + i] + factor * xi[c] : factor * xi[c];
+                f[c] = f[c] + add;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < DIMS; c++) R.F[c * n + i] = f[c];
+        if (MODE == 1) {
+#pragma unroll
+            for (int c = 0; c < DIMS; c++) {
+                double vc = R.v[c * n + i] + R.dt * f[c] / (2.0 * R.m);
+                R.v[c * n + i] = vc;
+                red[c] = vc;
+            }
+        } else if (MODE == 2) {
+#pragma unroll
+            for (int c = 0; c < DIMS; c++) red[c] = f[c];
+        }
+    }
+    if (MODE != 0) {
+        block_sum<3>(red);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int c = 0; c < 3; c++) R.part[t.local * 8 + c] = red[c];
+    }
+    if (arrive_last(dyn, CTR_FORCE, R.ntiles)) {
+        if (MODE != 0) {
+            double tot[3];
+            reduce_partials<3>(R, tot);
+            if (threadIdx.x == 0) {
+                for (int c = 0; c < DIMS; c++) {
+                    double mean = tot[c] / (double)n;
+                    if (MODE == 1)
+                        dyn->vmean[c] = mean;
+                    else
+                        dyn->Fmean[c] = mean;
+                }
+                if (MODE == 1) dyn->pending = 1 | (R.rescale ? 2 : 0);
+            }
+        }
+        if (threadIdx.x == 0) {
+            dyn->step = step + 1;
+            dyn->ctr[CTR_FORCE] = 0u;
+        }
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *reps, int sysmask)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (!((sysmask >> R.sys) & 1)) return;
+    const int key = (R.dims == 3 ? 4 : 0) + (R.pot == 2 ? 2 : 0) + (R.prec ? 1 : 0);
+    switch (key) {
+    case 0: force_tile<2, 1, 0, MODE>(R, t); break;
+    case 1: force_tile<2, 1, 1, MODE>(R, t); break;
+    case 2: force_tile<2, 2, 0, MODE>(R, t); break;
+    case 3: force_tile<2, 2, 1, MODE>(R, t); break;
+    case 4: force_tile<3, 1, 0, MODE>(R, t); break;
+    case 5: force_tile<3, 1, 1, MODE>(R, t); break;
+    case 6: force_tile<3, 2, 0, MODE>(R, t); break;
+    default: force_tile<3, 2, 1, MODE>(R, t); break;
+    }
+}
+
+// Kinetic energy of the COM-shifted velocities -> rescale factor
+// (src/MD.jl:542-553).  Runs only for replicas with rescale_temperature.
+template <int DIMS>
+__device__ void ekin_com_tile(const Rep &R, const Tile &t)
+{
+    RepDyn *dyn = R.dyn;
+    const int n = R.n, i = t.first + threadIdx.x;
+    double e[1] = {0.0};
+    if (i < n) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < DIMS; c++) {
+            double vc = R.v[c * n + i] - dyn->vmean[c];
+            s = (c == 0) ? vc * vc : s + vc * vc;
+        }
+        e[0] = s;
+    }
+    block_sum<1>(e);
+    if (threadIdx.x == 0) R.part[t.local * 8 + 4] = e[0];
+    if (arrive_last(dyn, CTR_EKCOM, R.ntiles)) {
+        double acc = 0.0;
+        for (int q = threadIdx.x; q < R.ntiles; q += SC_TB) acc += __builtin_nontemporal_load(&R.part[q * 8 + 4]);
+        double a[1] = {acc};
+        block_sum<1>(a);
+        if (threadIdx.x == 0) {
+            double ek = a[0] * (R.m / 2.0);
+            double kT = 2.0 * ek / ((double)DIMS * (double)n);
+            dyn->vscale = sqrt(R.kBT / kT);
+            dyn->ctr[CTR_EKCOM] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_ekin_com(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (!R.rescale || R.sys != 0) return;
+    if (R.dims == 3)
+        ekin_com_tile<3>(R, t);
+    else
+        ekin_com_tile<2>(R, t);
+}
+
+// Apply the pending COM shift / rescale so the state is reference-consistent
+// between API calls.
+__global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    RepDyn *dyn = R.dyn;
+    const int pend = dyn->pending;
+    const int n = R.n, i = t.first + threadIdx.x;
+    if (pend && i < n) {
+        for (int c = 0; c < R.dims; c++) {
+            double vc = R.v[c * n + i];
+            if (pend & 1) vc = vc - dyn->vmean[c];
+            if (pend & 2) vc = vc * dyn->vscale;
+            R.v[c * n + i] = vc;
+        }
+    }
+    if (arrive_last(dyn, CTR_MISC, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            dyn->pending = 0;
+            dyn->ctr[CTR_MISC] = 0u;
+        }
+    }
+}
+
+// ====================================================================
+// Neighbour-list rebuild (src/NeighborLists.jl:24-104): counting sort of the
+// slots by cell, then a stencil sweep writing the symmetric short list.
+// Every kernel returns at once for replicas whose rebuild flag is clear.
+// ====================================================================
+
+__global__ void k_request_rebuild(Rep *reps, int nreps)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nreps) reps[r].dyn->rebuild = 1;
+}
+
+// R1: cell of every slot and its arrival rank within the cell.
+__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (!R.dyn->rebuild) return;
+    const int i = t.first + threadIdx.x;
+    if (i >= R.n) return;
+    double4 p = R.pos[i];
+    int cc[3];
+    if (R.dims == 3)
+        cell_of_pos<3>(R, p, cc);
+    else
+        cell_of_pos<2>(R, p, cc);
+    int c = cc[0] + R.nc * (cc[1] + R.nc * cc[2]);
+    R.cell_of[i] = c;
+    R.rank[i] = atomicAdd(&R.cell_cnt[c], 1);
+}
+
+// R2: exclusive scan of the cell counts (one block per replica); clears the
+// counts for the next rebuild.
+__global__ __launch_bounds__(1024) void k_scan(const Rep *reps, int nreps)
+{
+    const int r = blockIdx.x;
+    if (r >= nreps) return;
+    const Rep &R = reps[r];
+    if (!R.dyn->rebuild) return;
+    __shared__ int sums[1024];
+    const int nt = blockDim.x, tid = threadIdx.x;
+    const int nc = R.ncell;
+    const int chunk = (nc + nt - 1) / nt;
+    const int b = tid * chunk, e = min(nc, b + chunk);
+    int s = 0;
+    for (int c = b; c < e; c++) s += R.cell_cnt[c];
+    sums[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < nt; o <<= 1) {
+        int v = (tid >= o) ? sums[tid - o] : 0;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    int run = sums[tid] - s;  // exclusive prefix of this chunk
+    for (int c = b; c < e; c++) {
+        int k = R.cell_cnt[c];
+        R.cell_start[c] = run;
+        R.cell_cnt[c] = 0;
+        run += k;
+    }
+    if (tid == nt - 1) R.cell_start[nc] = sums[nt - 1];
+}
+
+// R3: scatter old slots into cell order.
+__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (!R.dyn->rebuild) return;
+    const int i = t.first + threadIdx.x;
+    if (i >= R.n) return;
+    R.perm[R.cell_start[R.cell_of[i]] + R.rank[i]] = i;
+}
+
+// R4: order each cell's members by original index, so the new slot order
+// (and everything downstream) is a pure function of the positions.
+__global__ __launch_bounds__(SC_TB) void k_cellsort(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (!R.dyn->rebuild) return;
+    const int stride = R.ntiles * SC_TB;
+    for (int c = t.local * SC_TB + threadIdx.x; c < R.ncell; c += stride) {
+        const int b = R.cell_start[c], e = R.cell_start[c + 1];
+        for (int a = b + 1; a < e; a++) {
+            int x = R.perm[a];
+            int kx = R.orig[x];
+            int q = a - 1;
+            while (q >= b && R.orig[R.perm[q]] > kx) {
+                R.perm[q + 1] = R.perm[q];
+                q--;
+            }
+            R.perm[q + 1] = x;
+        }
+    }
+}
+
+// R5: gather the live state into the new slot order (alternate buffers).
+// F is dead at this point of the step (recomputed next) and disp is zeroed,
+// so only r/D, v and the original index move.
+__global__ __launch_bounds__(SC_TB) void k_permute(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (!R.dyn->rebuild) return;
+    const int k = t.first + threadIdx.x, n = R.n;
+    if (k >= n) return;
+    const int old = R.perm[k];
+    R.pos_alt[k] = R.pos[old];
+    for (int c = 0; c < R.dims; c++) R.v_alt[c * n + k] = R.v[c * n + old];
+    R.orig_alt[k] = R.orig[old];
+}
+
+// R6: stencil sweep.  For every slot: count the full-list partners
+// (d2 <= R_v^2, add_pair!'s caller condition) for the maxneighbours
+// overflow check and append each pair that also satisfies the half-list
+// predicate d2 < (r_cutoff*sigma_ij + skin)^2 (src/NeighborLists.jl:92) to
+// the symmetric short list.  Copies the permuted state back and zeroes the
+// displacements (src/NeighborLists.jl:35-42).
+template <int DIMS>
+__device__ void build_tile(const Rep &R, const Tile &t)
+{
+    const int n = R.n, k = t.first + threadIdx.x;
+    if (k < n) {
+        const double4 p = R.pos_alt[k];
+        int cc[3];
+        cell_of_pos<DIMS>(R, p, cc);
+        const int s = R.srange;
+        int nfull = 0, cnt = 0;
+        const int zr = DIMS == 3 ? s : 0;
+        for (int oz = -zr; oz <= zr; oz++)
+            for (int oy = -s; oy <= s; oy++)
+                for (int ox = -s; ox <= s; ox++) {
+                    const int c = wrapc(cc[0] + ox, R.nc) +
+                                  R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
+                    const int b = R.cell_start[c], e = R.cell_start[c + 1];
+                    for (int j = b; j < e; j++) {
+                        if (j == k) continue;
+                        const double4 q = R.pos_alt[j];
+                        double dx, dy, dz;
+                        double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
+                        if (d2 <= R.Rv2) {
+                            nfull++;
+                            double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
+                            double rh = R.rcut * md + R.skin;
+                            if (d2 < rh * rh) {
+                                if (cnt < R.cap) R.nbr[(size_t)cnt * n + k] = j;
+                                cnt++;
+                            }
+                        }
+                    }
+                }
+        if (nfull > R.cap) atomicOr(&R.dyn->err, SC_EBIT_OVERFLOW);
+        R.nn[k] = cnt < R.cap ? cnt : R.cap;
+        R.pos[k] = p;
+        for (int c = 0; c < DIMS; c++) {
+            R.v[c * n + k] = R.v_alt[c * n + k];
+            R.disp[c * n + k] = 0.0;
+        }
+        R.orig[k] = R.orig_alt[k];
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_build(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    RepDyn *dyn = R.dyn;
+    if (!dyn->rebuild) return;
+    if (R.dims == 3)
+        build_tile<3>(R, t);
+    else
+        build_tile<2>(R, t);
+    if (arrive_last(dyn, CTR_BUILD, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            dyn->rebuild = 0;
+            dyn->rebuilds += 1;
+            dyn->have_lists = 1;
+            dyn->maxdisp_bits = 0ull;
+            dyn->ctr[CTR_BUILD] = 0u;
+        }
+    }
+}
+
+// Full rows at the current positions over the rebuild-time cells, for the
+// reference-layout list export.
+template <int DIMS>
+__device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *counts, int cap)
+{
+    const int n = R.n, k = t.first + threadIdx.x;
+    if (k >= n) return;
+    const double4 p = R.pos[k];
+    int cc[3];
+    cell_of_pos<DIMS>(R, p, cc);
+    const int s = R.srange;
+    const int zr = DIMS == 3 ? s : 0;
+    int cnt = 0;
+    for (int oz = -zr; oz <= zr; oz++)
+        for (int oy = -s; oy <= s; oy++)
+            for (int ox = -s; ox <= s; ox++) {
+                const int c = wrapc(cc[0] + ox, R.nc) +
+                              R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
+                for (int j = R.cell_start[c]; j < R.cell_start[c + 1]; j++) {
+                    if (j == k) continue;
+                    double dx, dy, dz;
+                    double d2 = sqdist<DIMS>(R, p, R.pos[j], dx, dy, dz);
+                    if (d2 <= R.Rv2) {
+                        if (cnt < cap) rows[(size_t)R.orig[k] * cap + cnt] = R.orig[j];
+                        cnt++;
+                    }
+                }
+            }
+    counts[R.orig[k]] = cnt;
+}
+
+__global__ __launch_bounds__(SC_TB) void k_full_rows(const Tile *tiles, const Rep *reps, int *rows, int *counts, int cap)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (R.dims == 3)
+        full_rows_tile<3>(R, t, rows, counts, cap);
+    else
+        full_rows_tile<2>(R, t, rows, counts, cap);
+}
+
+// ====================================================================
+// Energies
+// ====================================================================
+
+// calculate_full_energy (src/AuxiliaryFunctions.jl:22-34): per-particle sums
+// over every candidate of the rebuild-time cell stencil with the inner loop's
+// cutoff test, which selects exactly the full-list pairs inside the cutoff;
+// Neumaier-compensated tile and replica reductions; total halved.
+template <int DIMS, int POT>
+__device__ void energy_tile(const Rep &R, const Tile &t)
+{
+    RepDyn *dyn = R.dyn;
+    const int n = R.n, k = t.first + threadIdx.x;
+    double e = 0.0;
+    if (k < n) {
+        const double4 p = R.pos[k];
+        int cc[3];
+        cell_of_pos<DIMS>(R, p, cc);
+        const int s = R.srange;
+        const int zr = DIMS == 3 ? s : 0;
+        for (int oz = -zr; oz <= zr; oz++)
+            for (int oy = -s; oy <= s; oy++)
+                for (int ox = -s; ox <= s; ox++) {
+                    const int c = wrapc(cc[0] + ox, R.nc) +
+                                  R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
+                    for (int j = R.cell_start[c]; j < R.cell_start[c + 1]; j++) {
+                        if (j == k) continue;
+                        const double4 q = R.pos[j];
+                        double dx, dy, dz;
+                        double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
+                        e += pair_energy<DIMS, POT>(R, d2, p.w, q.w);
+                    }
+                }
+    }
+    // compensated block reduction: exact two-sum tree is overkill here; a
+    // per-wave Neumaier pass over lane values keeps the error at O(eps).
+    __shared__ double ss[SC_TB];
+    ss[threadIdx.x] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0, comp = 0.0;
+        for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
+        R.part[t.local * 8 + 5] = sum;
+        R.part[t.local * 8 + 6] = comp;
+    }
+    __syncthreads();
+    if (arrive_last(dyn, CTR_ENERGY, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            double sum = 0.0, comp = 0.0;
+            for (int q = 0; q < R.ntiles; q++) {
+                neumaier_add(sum, comp, __builtin_nontemporal_load(&R.part[q * 8 + 5]));
+                comp += __builtin_nontemporal_load(&R.part[q * 8 + 6]);
+            }
+            dyn->result[0] = (sum + comp) / 2.0;
+            dyn->ctr[CTR_ENERGY] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_energy(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    const int key = (R.dims == 3 ? 2 : 0) + (R.pot == 2 ? 1 : 0);
+    switch (key) {
+    case 0: energy_tile<2, 1>(R, t); break;
+    case 1: energy_tile<2, 2>(R, t); break;
+    case 2: energy_tile<3, 1>(R, t); break;
+    default: energy_tile<3, 2>(R, t); break;
+    }
+}
+
+// calculate_full_energy_no_neigh (src/AuxiliaryFunctions.jl:86-110): all
+// pairs, j tiles staged through LDS.
+template <int DIMS, int POT>
+__device__ void energy_nn_tile(const Rep &R, const Tile &t)
+{
+    RepDyn *dyn = R.dyn;
+    __shared__ double4 sp[SC_TB];
+    const int n = R.n, k = t.first + threadIdx.x;
+    const double4 p = k < n ? R.pos[k] : make_double4(0, 0, 0, 1);
+    double e = 0.0;
+    for (int b = 0; b < n; b += SC_TB) {
+        __syncthreads();
+        if (b + threadIdx.x < n) sp[threadIdx.x] = R.pos[b + threadIdx.x];
+        __syncthreads();
+        const int m = min(SC_TB, n - b);
+        if (k < n)
+            for (int q = 0; q < m; q++) {
+                if (b + q == k) continue;
+                double dx, dy, dz;
+                double d2 = sqdist<DIMS>(R, p, sp[q], dx, dy, dz);
+                e += pair_energy_generic<POT>(R, d2, p.w, sp[q].w);
+            }
+    }
+    __shared__ double ss[SC_TB];
+    ss[threadIdx.x] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0, comp = 0.0;
+        for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
+        R.part[t.local * 8 + 5] = sum;
+        R.part[t.local * 8 + 6] = comp;
+    }
+    __syncthreads();
+    if (arrive_last(dyn, CTR_ENERGY, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            double sum = 0.0, comp = 0.0;
+            for (int q = 0; q < R.ntiles; q++) {
+                neumaier_add(sum, comp, __builtin_nontemporal_load(&R.part[q * 8 + 5]));
+                comp += __builtin_nontemporal_load(&R.part[q * 8 + 6]);
+            }
+            dyn->result[0] = (sum + comp) / 2.0;
+            dyn->ctr[CTR_ENERGY] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_energy_nn(const Tile *tiles, const Rep *reps)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    const int key = (R.dims == 3 ? 2 : 0) + (R.pot == 2 ? 1 : 0);
+    switch (key) {
+    case 0: energy_nn_tile<2, 1>(R, t); break;
+    case 1: energy_nn_tile<2, 2>(R, t); break;
+    case 2: energy_nn_tile<3, 1>(R, t); break;
+    default: energy_nn_tile<3, 2>(R, t); break;
+    }
+}
+
+// find_kinetic_energy (src/AuxiliaryFunctions.jl:57-65); with apply != 0 it
+// also stages rescale_velocities! (src/MD.jl:391-402) as a pending scale.
+__global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *reps, int apply)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    RepDyn *dyn = R.dyn;
+    const int n = R.n, i = t.first + threadIdx.x;
+    double e[1] = {0.0};
+    if (i < n) {
+        double s = 0.0;
+        for (int c = 0; c < R.dims; c++) {
+            double vc = R.v[c * n + i];
+            s = (c == 0) ? vc * vc : s + vc * vc;
+        }
+        e[0] = s;
+    }
+    block_sum<1>(e);
+    if (threadIdx.x == 0) R.part[t.local * 8 + 7] = e[0];
+    if (arrive_last(dyn, CTR_EKIN, R.ntiles)) {
+        double acc = 0.0;
+        for (int q = threadIdx.x; q < R.ntiles; q += SC_TB) acc += __builtin_nontemporal_load(&R.part[q * 8 + 7]);
+        double a[1] = {acc};
+        block_sum<1>(a);
+        if (threadIdx.x == 0) {
+            double ek = a[0] * (R.m / 2.0);
+            dyn->result[1] = ek;
+            if (apply) {
+                double kT = 2.0 * ek / ((double)R.dims * (double)n);
+                dyn->vscale = sqrt(R.kBT / kT);
+                dyn->pending = 2;
+            }
+            dyn->ctr[CTR_EKIN] = 0u;
+        }
+    }
+}
+
+// ====================================================================
+// Host-order transfers: staging buffer indexed by original particle index.
+// ====================================================================
+__global__ __launch_bounds__(SC_TB) void k_to_host(const Tile *tiles, const Rep *reps, int field)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    const int n = R.n, k = t.first + threadIdx.x, d = R.dims;
+    if (k >= n) return;
+    const int o = R.orig[k];
+    double *st = R.stage;
+    switch (field) {
+    case FLD_R: {
+        double4 p = R.pos[k];
+        st[(size_t)o * d + 0] = p.x;
+        st[(size_t)o * d + 1] = p.y;
+        if (d == 3) st[(size_t)o * d + 2] = p.z;
+    } break;
+    case FLD_D: st[o] = R.pos[k].w; break;
+    case FLD_V:
+    case FLD_F:
+    case FLD_DISP: {
+        const double *src = field == FLD_V ? R.v : (field == FLD_F ? R.F : R.disp);
+        for (int c = 0; c < d; c++) st[(size_t)o * d + c] = src[c * n + k];
+    } break;
+    case FLD_DISP2: {
+        double s = 0.0;
+        for (int c = 0; c < d; c++) {
+            double x = R.disp[c * n + k];
+            s = (c == 0) ? x * x : s + x * x;
+        }
+        st[o] = s;
+    } break;
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Rep *reps, int field)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    const int n = R.n, k = t.first + threadIdx.x, d = R.dims;
+    if (k >= n) return;
+    const int o = R.orig[k];
+    const double *st = R.stage;
+    switch (field) {
+    case FLD_R: {
+        double4 p = R.pos[k];
+        p.x = st[(size_t)o * d + 0];
+        p.y = st[(size_t)o * d + 1];
+        p.z = d == 3 ? st[(size_t)o * d + 2] : 0.0;
+        R.pos[k] = p;
+    } break;
+    case FLD_D: {
+        double4 p = R.pos[k];
+        p.w = st[o];
+        R.pos[k] = p;
+    } break;
+    case FLD_V:
+    case FLD_F:
+    case FLD_DISP: {
+        double *dst = field == FLD_V ? R.v : (field == FLD_F ? R.F : R.disp);
+        for (int c = 0; c < d; c++) dst[c * n + k] = st[(size_t)o * d + c];
+    } break;
+    }
+}
+
+// ====================================================================
+// Capped steepest descent (find_inherent_structure,
+// src/InherentStructure.jl:26-51)
+// ====================================================================
+__global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Rep *reps, double dt, double max_step)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    RepDyn *dyn = R.dyn;
+    const int n = R.n, i = t.first + threadIdx.x;
+    double st = 0.0, fm = 0.0;
+    if (i < n) {
+        double s = 0.0, f2 = 0.0;
+        for (int c = 0; c < R.dims; c++) {
+            double fc = R.F[c * n + i];
+            double vc = fc * dt;
+            s = (c == 0) ? vc * vc : s + vc * vc;
+            f2 = (c == 0) ? fc * fc : f2 + fc * fc;
+        }
+        st = sqrt(s);
+        fm = sqrt(f2);
+    }
+    double m1 = block_max(st);
+    double m2 = block_max(fm);
+    if (threadIdx.x == 0) {
+        atomic_max_nonneg(&dyn->maxaux_bits, m1);
+        R.part[t.local * 8 + 3] = m2;
+    }
+    if (arrive_last(dyn, CTR_MISC, R.ntiles)) {
+        double mf = 0.0;
+        for (int q = threadIdx.x; q < R.ntiles; q += SC_TB) mf = fmax(mf, __builtin_nontemporal_load(&R.part[q * 8 + 3]));
+        mf = block_max(mf);
+        if (threadIdx.x == 0) {
+            double stepsize = __longlong_as_double((long long)atomicAdd(&dyn->maxaux_bits, 0ull));
+            dyn->result[2] = (stepsize > max_step) ? max_step / stepsize : -1.0;  // -1: unscaled
+            dyn->result[3] = mf;
+            dyn->maxaux_bits = 0ull;
+            dyn->ctr[CTR_MISC] = 0u;
+        }
+    }
+}
+
+template <int DIMS>
+__device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
+{
+    RepDyn *dyn = R.dyn;
+    const int n = R.n, i = t.first + threadIdx.x;
+    const double scale = dyn->result[2];
+    double s2 = 0.0;
+    if (i < n) {
+        double4 p = R.pos[i];
+        double pc[3] = {p.x, p.y, p.z};
+        double s = 0.0;
+        bool fin = true;
+#pragma unroll
+        for (int c = 0; c < DIMS; c++) {
+            double vc = R.F[c * n + i] * dt;
+            if (scale >= 0.0) vc = vc * scale;
+            R.v[c * n + i] = vc;
+            double dc = R.disp[c * n + i] + vc;
+            s = (c == 0) ? dc * dc : s + dc * dc;
+            double rn = pc[c] + vc;
+            rn -= floor(rn / R.L) * R.L;
+            pc[c] = rn;
+            fin = fin && isfinite(rn);
+            R.disp[c * n + i] = dc;
+        }
+        R.pos[i] = make_double4(pc[0], pc[1], pc[2], p.w);
+        s2 = s;
+        if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
+    }
+    double m = block_max(s2);
+    if (threadIdx.x == 0) atomic_max_nonneg(&dyn->maxdisp_bits, m);
+    if (arrive_last(dyn, CTR_DRIFT, R.ntiles)) {
+        if (threadIdx.x == 0) {
+            double mx = __longlong_as_double((long long)atomicAdd(&dyn->maxdisp_bits, 0ull));
+            if (mx > R.thr) dyn->rebuild = 1;
+            dyn->maxdisp_bits = 0ull;
+            dyn->ctr[CTR_DRIFT] = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_min_move(const Tile *tiles, const Rep *reps, double dt)
+{
+    const Tile t = tiles[blockIdx.x];
+    const Rep &R = reps[t.rep];
+    if (R.dims == 3)
+        min_move_tile<3>(R, t, dt);
+    else
+        min_move_tile<2>(R, t, dt);
+}
+
+inline int check(hipError_t e) { return e == hipSuccess ? 0 : -5; }
+
+}  // namespace
+
+// ====================================================================
+// Launchers
+// ====================================================================
+namespace sc {
+
+#define LAUNCH(kern, grid, block, s, ...)                                 \
+    do {                                                                 \
+        if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__); \
+        return check(hipGetLastError());                                 \
+    } while (0)
+
+int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    LAUNCH(k_kick_drift, ntiles, SC_TB, s, tiles, reps);
+}
+
+int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int mode, int sysmask, hipStream_t s)
+{
+    if (mode == 0) LAUNCH(k_force<0>, ntiles, SC_TB, s, tiles, reps, sysmask);
+    if (mode == 1) LAUNCH(k_force<1>, ntiles, SC_TB, s, tiles, reps, sysmask);
+    LAUNCH(k_force<2>, ntiles, SC_TB, s, tiles, reps, sysmask);
+}
+
+int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    LAUNCH(k_drift_brownian, ntiles, SC_TB, s, tiles, reps);
+}
+
+int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    LAUNCH(k_ekin_com, ntiles, SC_TB, s, tiles, reps);
+}
+
+int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s)
+{
+    LAUNCH(k_request_rebuild, (nreps + 255) / 256, 256, s, const_cast<Rep *>(reps), nreps);
+}
+
+int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_bin, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_scan, dim3(nreps), dim3(1024), 0, s, reps, nreps);
+    hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_cellsort, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_permute, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_build, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    return check(hipGetLastError());
+}
+
+int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    LAUNCH(k_finalize, ntiles, SC_TB, s, tiles, reps);
+}
+
+int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    LAUNCH(k_energy, ntiles, SC_TB, s, tiles, reps);
+}
+
+int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    LAUNCH(k_energy_nn, ntiles, SC_TB, s, tiles, reps);
+}
+
+int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int apply_rescale, hipStream_t s)
+{
+    LAUNCH(k_ekin, ntiles, SC_TB, s, tiles, reps, apply_rescale);
+}
+
+int launch_from_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s)
+{
+    LAUNCH(k_from_host, ntiles, SC_TB, s, tiles, reps, field);
+}
+
+int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s)
+{
+    LAUNCH(k_to_host, ntiles, SC_TB, s, tiles, reps, field);
+}
+
+int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, int *counts, int cap, hipStream_t s)
+{
+    LAUNCH(k_full_rows, ntiles, SC_TB, s, tiles, reps, rows, counts, cap);
+}
+
+int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, double dt, double max_step, hipStream_t s)
+{
+    LAUNCH(k_min_scale, ntiles, SC_TB, s, tiles, reps, dt, max_step);
+}
+
+int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, double dt, hipStream_t s)
+{
+    LAUNCH(k_min_move, ntiles, SC_TB, s, tiles, reps, dt);
+}
+
+void philox_host(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]) { philox4x32_10(in, key, out); }
+
+}  // namespace sc
