@@ -1,0 +1,207 @@
+"""Benchmark: particle-steps/s of the MD hot path on one MI355X.
+
+Workload (default): 3D polydisperse non-additive soft spheres, N = 10^6,
+rho = 1.02, skin 0.3, Newtonian NVE (velocity Verlet, dt = 0.005) from a
+seeded jittered simple-cubic lattice, relaxed on the device by capped
+steepest descent and thermalised at kBT = 0.2 by velocity rescaling (all
+untimed).  Inputs are resident in HBM when the timed region starts.
+
+The timed region is K calls' worth of do_time_step (neighbour rebuilds on
+the skin trigger included) in one sc_do_time_steps call, bracketed by device
+synchronisation.  `value` is the FP64 drop-in form; the mixed-precision form
+(FP32 pair arithmetic) is measured on the same state and reported beside it
+with its max relative force error and NVE drift.
+
+roofline: the dominant kernel is the pair-force kernel (fused with the
+second half-kick and the COM partial sums).  Its algorithmic bytes per
+particle are r (24) + D (8) + v read/write (48) + F write (24) + 4 (n_half+1)
+neighbour-index bytes, n_half measured from the device lists; its average
+duration comes from HIP events on the context stream over the timed region.
+
+cpu_baseline: the oracle (FP64 C restatement of the reference's serial
+algorithm, one thread) on the same N = 10^6 state, timed for a bounded number
+of steps (~15 s), rebuilds included.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1):
+    from simulationcode_amd import md
+    U = md.Berthier(rc=1.25)
+    p = nat.ScParams()
+    p.dims, p.potential, p.system, p.precision = 3, nat.POT_BERTHIER, nat.SYS_NEWTONIAN, precision
+    p.rescale_temperature, p.cell_divisor = 0, 1
+    p.N, p.maxneighbours = N, 200
+    p.box_size, p.force_cutoff, p.force_cutoff2, p.skin_distance = L, 1.25, 1.25 ** 2, 0.3
+    p.b_c0, p.b_c2, p.b_c4, p.b_zeta = U.c0, U.c2, U.c4, U.ζ
+    p.kBT, p.gamma, p.dt, p.m, p.seed = kBT, 1.0, dt, 1.0, seed
+    return nat.Context(p, D), p
+
+
+def total_energy(ctx):
+    return ctx.full_energy() + ctx.kinetic_energy()
+
+
+def timed_run(ctx, steps, warmup):
+    ctx.do_time_steps(warmup)
+    e0 = total_energy(ctx)
+    reb0 = ctx.rebuild_count()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    ctx.do_time_steps(steps)  # synchronises on return
+    wall = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    dev_ms = ctx.last_elapsed_ms()
+    e1 = total_energy(ctx)
+    return {"wall_s": wall, "device_ms": dev_ms, "e0": e0, "e1": e1, "rebuilds": ctx.rebuild_count() - reb0,
+            "kt": kt}
+
+
+def cpu_baseline(op_state, budget_s=15.0):
+    import oracle
+    r, v, F, D, L, N = op_state
+    op = oracle.make_params(dims=3, N=N, box_size=L, force_cutoff=1.25, skin=0.3, D=D,
+                            potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN, dt=0.005, kBT=0.2,
+                            maxneighbours=128)
+    st = oracle.State(op, r, v, D, F=F)
+    st.update_neighbor_lists(cells=True)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        st.do_time_steps(1, cells=True)
+        steps += 1
+        el = time.perf_counter() - t0
+        if (el > budget_s and steps >= 3) or steps >= 2000:
+            break
+    return {"value": N * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (serial FP64 C restatement, cell-list rebuild) on the same N={N} state, "
+                      f"{steps} steps in {el:.1f} s incl. {int(st.rebuilds[0]) - 1} rebuilds"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--N", type=int, default=10 ** 6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261206)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the hot path would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from simulationcode_amd import _native as nat, synth
+    nat.load()
+    N = args.N
+    rho, dims = 1.02, 3
+    L = synth.box_size(N, rho, dims)
+    rng = np.random.default_rng(args.seed)
+    D = synth.berthier_diameters(N, rng)
+    r = synth.lattice_positions(N, dims, L, 0.1, rng)
+
+    log(f"[bench] setup N={N} L={L:.3f}")
+    ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64)
+    ctx.set_state(r=r, v=np.zeros((N, dims)), F=np.zeros((N, dims)))
+    ctx.update_neighbor_lists()
+    t0 = time.perf_counter()
+    fmax = ctx.minimize(400, 0.005, 0.02)
+    log(f"[bench] relaxed: max|F| = {fmax:.3g} ({time.perf_counter() - t0:.1f}s)")
+    v = synth.maxwell_boltzmann(N, dims, 0.2, 1.0, rng)
+    ctx.set_state(v=v, F=np.zeros((N, dims)))
+    ctx.update_neighbor_lists()
+    ctx.calculate_new_forces(None)
+    for _ in range(10):
+        ctx.do_time_steps(50)
+        ctx.rescale_velocities()
+    log("[bench] thermalised at kBT=0.2")
+    state = ctx.get_state()
+    tot, mx = ctx.neighbor_stats()
+    n_half = tot / 2.0 / N
+
+    res64 = timed_run(ctx, args.steps, args.warmup)
+    log(f"[bench] fp64: {res64['device_ms'] / args.steps * 1e3:.1f} us/step, rebuilds {res64['rebuilds']}")
+    ms64 = res64["device_ms"] / args.steps
+    value = N * args.steps / (res64["device_ms"] / 1e3)
+
+    # roofline of the pair-force kernel
+    kt = res64["kt"]
+    f_avg_s = kt["force_ms"] / max(kt["force_launches"], 1) / 1e3
+    bytes_per_particle = 24 + 8 + 48 + 24 + 4 * (n_half + 1)
+    achieved = N * bytes_per_particle / f_avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_force<1> (pair forces + second half-kick + COM partials)",
+                "kernel_avg_us": round(f_avg_s * 1e6, 2), "bytes_per_particle": round(bytes_per_particle, 2),
+                "n_half_mean": round(n_half, 3), "drift_kernel_avg_us":
+                    round(kt["drift_ms"] / max(kt["drift_launches"], 1) * 1e3, 2)}
+
+    drift64 = (res64["e1"] - res64["e0"]) / abs(res64["e0"]) * (1e5 / args.steps)
+    out = {"metric": "particle-steps/s FP64 vs mixed @ stated force err; p50/p99 replica latency",
+           "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded jittered lattice, relaxed and thermalised on device)",
+           "config": {"workload": f"3D polydisperse non-additive soft spheres, N={N}, rho=1.02, Newtonian NVE "
+                                  f"dt=0.005, skin 0.3, kBT=0.2 start", "N": N, "dims": 3,
+                      "potential": "Berthier (rc=1.25, zeta=0.2)", "dynamics": "Newtonian", "dt": 0.005},
+           "roofline": roofline,
+           "fp64": {"rebuilds": res64["rebuilds"], "nve_rel_drift_per_1e5_steps": drift64,
+                    "wall_s": res64["wall_s"]}}
+
+    if not args.no_mixed:
+        ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED)
+        ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
+        ctxm.update_neighbor_lists()
+        ctx.set_state(r=state["r"], v=state["v"], F=state["F"])
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        ctxm.calculate_new_forces(None)
+        F64 = ctx.get_state(F=True, r=False, v=False)["F"]
+        F32 = ctxm.get_state(F=True, r=False, v=False)["F"]
+        fn = np.linalg.norm(F64, axis=1)
+        dn = np.linalg.norm(F32 - F64, axis=1)
+        ctx.close()
+        resm = timed_run(ctxm, args.steps, args.warmup)
+        msm = resm["device_ms"] / args.steps
+        kmt = resm["kt"]
+        out["mixed"] = {"value": N * args.steps / (resm["device_ms"] / 1e3), "ms_per_step": msm,
+                        "speedup_vs_fp64": ms64 / msm,
+                        "max_rel_force_err_vs_mean": float(dn.max() / fn.mean()),
+                        "max_rel_force_err_vs_max": float(dn.max() / fn.max()),
+                        "nve_rel_drift_per_1e5_steps": (resm["e1"] - resm["e0"]) / abs(resm["e0"]) * (1e5 / args.steps),
+                        "rebuilds": resm["rebuilds"],
+                        "force_kernel_avg_us": kmt["force_ms"] / max(kmt["force_launches"], 1) * 1e3}
+        log(f"[bench] mixed: {msm * 1e3:.1f} us/step")
+        ctxm.close()
+    else:
+        ctx.close()
+
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle, 1 thread)")
+        out["cpu_baseline"] = cpu_baseline((state["r"], state["v"], state["F"], D, L, N))
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -140,6 +140,19 @@ void sc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
  * context stream), in milliseconds. */
 int sc_last_elapsed_ms(sc_ctx *ctx, double *ms);
 
+/* Measurement hooks (no reference counterpart; the reference only wraps its
+ * loop in @time, src/MD.jl:704).  With timing on, sc_do_time_steps brackets
+ * every pair-force launch and every drift launch with HIP events on the
+ * context stream; sc_kernel_times returns the accumulated milliseconds and
+ * launch counts since timing was switched on. */
+int sc_set_kernel_timing(sc_ctx *ctx, int on);
+int sc_kernel_times(sc_ctx *ctx, double *force_ms, int64_t *force_launches, double *drift_ms,
+                    int64_t *drift_launches);
+
+/* Short-list statistics of the current lists: total entries (each pair
+ * counted from both sides) and the longest row. */
+int sc_neighbor_stats(sc_ctx *ctx, int64_t *total_entries, int64_t *max_row);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,7 +43,7 @@ EXPORTS = (
     "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_calculate_new_forces",
     "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
     "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_philox4x32_10",
-    "sc_last_elapsed_ms",
+    "sc_last_elapsed_ms", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
 )
 
 
@@ -122,6 +122,9 @@ def load(path=LIB_PATH):
         "sc_minimize": (ctypes.c_int, [vp, i64, dbl, dbl, pd]),
         "sc_philox4x32_10": (None, [vp, vp, vp]),
         "sc_last_elapsed_ms": (ctypes.c_int, [vp, pd]),
+        "sc_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "sc_kernel_times": (ctypes.c_int, [vp, pd, ctypes.POINTER(i64), pd, ctypes.POINTER(i64)]),
+        "sc_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -268,3 +271,19 @@ class Context:
 
     def last_elapsed_ms(self):
         return self._scalar(self._lib.sc_last_elapsed_ms)
+
+    def set_kernel_timing(self, on):
+        self._check(self._lib.sc_set_kernel_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        fm, dm = ctypes.c_double(), ctypes.c_double()
+        fn, dn = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.sc_kernel_times(self._h, ctypes.byref(fm), ctypes.byref(fn), ctypes.byref(dm),
+                                              ctypes.byref(dn)))
+        return {"force_ms": fm.value, "force_launches": fn.value, "drift_ms": dm.value,
+                "drift_launches": dn.value}
+
+    def neighbor_stats(self):
+        t, m = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.sc_neighbor_stats(self._h, ctypes.byref(t), ctypes.byref(m)))
+        return t.value, m.value

@@ -32,6 +32,10 @@ struct sc_ctx {
     long long host_step = 0;
     double last_ms = 0.0;
     std::vector<void *> allocs;
+    bool timing = false;
+    std::vector<hipEvent_t> kev;
+    double t_force = 0.0, t_drift = 0.0;
+    long long n_force = 0, n_drift = 0;
 };
 
 namespace {
@@ -140,6 +144,16 @@ int set_noise(sc_ctx *c, const double *noise, int64_t blocks)
     c->h.noise = c->d_noise;
     c->h.noise_step0 = c->host_step;
     return push_rep(c);
+}
+
+int ensure_events(sc_ctx *c, size_t n)
+{
+    while (c->kev.size() < n) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->kev.push_back(e);
+    }
+    return SC_OK;
 }
 
 }  // namespace
@@ -275,6 +289,7 @@ void sc_destroy(sc_ctx *c)
     if (c->s) (void)hipStreamSynchronize(c->s);
     for (void *q : c->allocs) (void)hipFree(q);
     if (c->d_noise) (void)hipFree(c->d_noise);
+    for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -395,16 +410,26 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
     if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, nsteps))) return rc;
     const int nt = c->ntiles;
+    const bool timed = c->timing;
+    if (timed && (rc = ensure_events(c, 4 * (size_t)nsteps))) return rc;
+    size_t ev = 0;
     HIPCHK(c, hipEventRecord(c->ev0, c->s));
     for (int64_t s = 0; s < nsteps; s++) {
         if (c->h.sys == SC_SYS_BROWNIAN) {
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 2, 7, c->s));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, c->s));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
         } else {
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_kick_drift(c->d_tiles, nt, c->d_rep, c->s));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 7, c->s));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, c->s));
         }
     }
@@ -415,6 +440,26 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     c->host_step += nsteps;
+    if (timed) {
+        // Newtonian/Langevin: [drift][rebuild][force]; Brownian: [force][drift][rebuild]
+        for (size_t q = 0; q < ev; q += 3) {
+            float a = 0.f, b = 0.f;
+            if (c->h.sys == SC_SYS_BROWNIAN) {
+                HIPCHK(c, hipEventElapsedTime(&a, c->kev[q], c->kev[q + 1]));
+                HIPCHK(c, hipEventElapsedTime(&b, c->kev[q + 1], c->kev[q + 2]));
+                c->t_force += a;
+                c->t_drift += b;
+            } else {
+                HIPCHK(c, hipEventElapsedTime(&b, c->kev[q], c->kev[q + 1]));
+                HIPCHK(c, hipEventElapsedTime(&a, c->kev[q + 2], c->kev[q + 3]));
+                c->t_force += a;
+                c->t_drift += b;
+                q += 1;
+            }
+            c->n_force++;
+            c->n_drift++;
+        }
+    }
     return check_dyn(c);
 }
 
@@ -508,6 +553,41 @@ int sc_last_elapsed_ms(sc_ctx *c, double *ms)
 {
     if (!c || !ms) return SC_ERR_ARG;
     *ms = c->last_ms;
+    return SC_OK;
+}
+
+int sc_set_kernel_timing(sc_ctx *c, int on)
+{
+    if (!c) return SC_ERR_ARG;
+    c->timing = on != 0;
+    c->t_force = c->t_drift = 0.0;
+    c->n_force = c->n_drift = 0;
+    return SC_OK;
+}
+
+int sc_kernel_times(sc_ctx *c, double *force_ms, int64_t *force_launches, double *drift_ms, int64_t *drift_launches)
+{
+    if (!c) return SC_ERR_ARG;
+    if (force_ms) *force_ms = c->t_force;
+    if (force_launches) *force_launches = c->n_force;
+    if (drift_ms) *drift_ms = c->t_drift;
+    if (drift_launches) *drift_launches = c->n_drift;
+    return SC_OK;
+}
+
+int sc_neighbor_stats(sc_ctx *c, int64_t *total, int64_t *max_row)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    std::vector<int> nn(c->h.n);
+    HIPCHK(c, hipMemcpyAsync(nn.data(), c->h.nn, sizeof(int) * nn.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    int64_t t = 0, m = 0;
+    for (int x : nn) {
+        t += x;
+        m = std::max<int64_t>(m, x);
+    }
+    if (total) *total = t;
+    if (max_row) *max_row = m;
     return SC_OK;
 }
 

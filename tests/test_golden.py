@@ -1,0 +1,62 @@
+"""Golden fixtures (tests/golden/*.npz, written by make_golden.py):
+the oracle must reproduce them exactly (CPU), and the GPU must match them to
+the parity bars (GPU)."""
+import numpy as np
+import pytest
+
+import oracle
+from simulationcode_amd import _native as nat
+from tests import helpers as H
+from tests.golden import make_golden as G
+
+NAMES = [f[0] for f in G.FIXTURES]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_oracle_reproduces_fixture(name):
+    d = G.load(name)
+    op = G.params_of(d)
+    noise = d.get("noise")
+    lists = oracle.build_lists(op, d["r"], d["D"], cells=True)
+    assert np.array_equal(lists.full_pairs(), d["full_pairs"])
+    assert np.array_equal(lists.half_pairs(), d["half_pairs"])
+    lists = oracle.build_lists(op, d["r"], d["D"], cells=False)
+    F0 = oracle.calculate_new_forces(op, d["r"], d["D"], lists, d["v"], None if noise is None else noise[0])
+    assert np.array_equal(F0, d["F0"])
+    assert oracle.full_energy(op, d["r"], d["D"], lists) == d["energy"]
+    assert oracle.energy_no_neigh(op, d["r"], d["D"]) == d["energy_nn"]
+    st = oracle.State(op, d["r"], d["v"], d["D"], F=F0)
+    st.update_neighbor_lists(cells=False)
+    st.do_time_steps(int(d["steps"]), None if noise is None else noise[1:], cells=False)
+    assert np.array_equal(st.r, d["r_end"]) and np.array_equal(st.v, d["v_end"])
+    assert st.rebuilds[0] == d["rebuilds"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", NAMES)
+def test_gpu_matches_fixture(name):
+    d = G.load(name)
+    op = G.params_of(d)
+    noise = d.get("noise")
+    p = H.sc_params_from(op, d["params"]["force_cutoff"])
+    ctx = nat.Context(p, d["D"])
+    ctx.set_state(r=d["r"], v=d["v"])
+    ctx.update_neighbor_lists()
+    nf, full, nh, half = ctx.get_neighbor_lists()
+    assert np.array_equal(oracle._pairs(nf, full - 1, both=True), d["full_pairs"])
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), d["half_pairs"])
+    ctx.calculate_new_forces(None if noise is None else noise[0])
+    Fg = ctx.get_state(F=True)["F"]
+    assert H.force_rel_err(Fg, d["F0"]) <= 1e-12
+    e = ctx.full_energy()
+    assert abs(e - d["energy"]) <= 1e-12 * max(1.0, abs(d["energy"]))
+    ctx.set_state(F=d["F0"])
+    ctx.rebuild_count(0)
+    ctx.update_neighbor_lists()
+    ctx.do_time_steps(int(d["steps"]), None if noise is None else noise[1:])
+    g = ctx.get_state()
+    dr = g["r"] - d["r_end"]
+    dr -= np.round(dr / op.box_size) * op.box_size
+    assert np.max(np.abs(dr)) <= 1e-10
+    assert np.max(np.abs(g["v"] - d["v_end"])) <= 1e-10
+    assert ctx.rebuild_count() == d["rebuilds"]

@@ -1,0 +1,57 @@
+"""CPU checks of the boundary: libsc loads and exports every symbol the public
+header declares; argument validation that returns before any device call;
+the noise generator's host twin reproduces the published Philox4x32-10
+known-answer vectors."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from simulationcode_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "sc.h")).read()
+    return sorted(set(re.findall(r"\b(sc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(nat.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(nat.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+def test_params_struct_matches_header_size():
+    # sc_params: 6 int32, 2 int64, 4 + 3 + 4 + 4 doubles, one uint64
+    assert ctypes.sizeof(nat.ScParams) == 6 * 4 + 2 * 8 + 15 * 8 + 8
+
+
+def test_create_rejects_bad_arguments_without_touching_the_device():
+    p = nat.ScParams()
+    p.dims = 4
+    p.N = 10
+    with pytest.raises(nat.ScError) as e:
+        nat.Context(p, np.ones(10))
+    assert e.value.code == nat.SC_ERR_ARG
+
+
+# Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11)
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", KAT)
+def test_philox_known_answers(ctr, key, out):
+    assert tuple(int(x) for x in nat.philox4x32_10(ctr, key)) == out
