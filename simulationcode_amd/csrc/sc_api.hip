@@ -395,7 +395,7 @@ int sc_calculate_new_forces(sc_ctx *c, const double *noise)
     if (rc) return rc;
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
     if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, 1))) return rc;
-    KCHK(c, sc::launch_force(c->d_tiles, c->ntiles, c->d_rep, 0, 7, c->s));
+    KCHK(c, sc::launch_force(c->d_tiles, c->ntiles, c->d_rep, 1, 0, 7, c->s));
     c->host_step += 1;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return SC_OK;
@@ -417,23 +417,25 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     for (int64_t s = 0; s < nsteps; s++) {
         if (c->h.sys == SC_SYS_BROWNIAN) {
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 2, 7, c->s));
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 2, 7, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, c->s));
+            KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, 1, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
         } else {
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_kick_drift(c->d_tiles, nt, c->d_rep, c->s));
+            KCHK(c, sc::launch_kick_drift_only(c->d_tiles, nt, c->d_rep, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
+            KCHK(c, sc::launch_drift_finish(c->d_rep, 1, 3, c->s));
             KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 7, c->s));
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 1, 7, c->s, false));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, c->s));
+            KCHK(c, sc::launch_force_finish(c->d_rep, 1, 1, 7, c->s));
+            if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, 1, c->s));
         }
     }
-    KCHK(c, sc::launch_finalize(c->d_tiles, nt, c->d_rep, c->s));
+    KCHK(c, sc::launch_finalize(c->d_tiles, nt, c->d_rep, 1, c->s));
     HIPCHK(c, hipEventRecord(c->ev1, c->s));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -470,7 +472,7 @@ int sc_full_energy(sc_ctx *c, double *out)
     int rc = read_dyn(c, &d);
     if (rc) return rc;
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
-    KCHK(c, sc::launch_energy(c->d_tiles, c->ntiles, c->d_rep, c->s));
+    KCHK(c, sc::launch_energy(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
     if ((rc = read_dyn(c, &d))) return rc;
     *out = d.result[0];
     return SC_OK;
@@ -479,7 +481,7 @@ int sc_full_energy(sc_ctx *c, double *out)
 int sc_energy_no_neigh(sc_ctx *c, double *out)
 {
     if (!c || !c->d_rep || !out) return SC_ERR_ARG;
-    KCHK(c, sc::launch_energy_nn(c->d_tiles, c->ntiles, c->d_rep, c->s));
+    KCHK(c, sc::launch_energy_nn(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
     RepDyn d;
     int rc = read_dyn(c, &d);
     if (rc) return rc;
@@ -490,7 +492,7 @@ int sc_energy_no_neigh(sc_ctx *c, double *out)
 int sc_kinetic_energy(sc_ctx *c, double *out)
 {
     if (!c || !c->d_rep || !out) return SC_ERR_ARG;
-    KCHK(c, sc::launch_ekin(c->d_tiles, c->ntiles, c->d_rep, 0, c->s));
+    KCHK(c, sc::launch_ekin(c->d_tiles, c->ntiles, c->d_rep, 1, 0, c->s));
     RepDyn d;
     int rc = read_dyn(c, &d);
     if (rc) return rc;
@@ -501,8 +503,8 @@ int sc_kinetic_energy(sc_ctx *c, double *out)
 int sc_rescale_velocities(sc_ctx *c)
 {
     if (!c || !c->d_rep) return SC_ERR_ARG;
-    KCHK(c, sc::launch_ekin(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
-    KCHK(c, sc::launch_finalize(c->d_tiles, c->ntiles, c->d_rep, c->s));
+    KCHK(c, sc::launch_ekin(c->d_tiles, c->ntiles, c->d_rep, 1, 1, c->s));
+    KCHK(c, sc::launch_finalize(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
     return SC_OK;
 }
@@ -536,9 +538,9 @@ int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *ma
     c->h.sys = SC_SYS_NEWTONIAN;
     if ((rc = push_rep(c))) return rc;
     for (int64_t it = 0; it < iters; it++) {
-        KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 0, 7, c->s));
-        KCHK(c, sc::launch_min_scale(c->d_tiles, nt, c->d_rep, dt, max_step, c->s));
-        KCHK(c, sc::launch_min_move(c->d_tiles, nt, c->d_rep, dt, c->s));
+        KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 0, 7, c->s));
+        KCHK(c, sc::launch_min_scale(c->d_tiles, nt, c->d_rep, 1, dt, max_step, c->s));
+        KCHK(c, sc::launch_min_move(c->d_tiles, nt, c->d_rep, 1, dt, c->s));
         KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
     }
     c->host_step += iters;

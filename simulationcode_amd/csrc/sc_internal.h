@@ -80,23 +80,29 @@ struct Tile {
     int pad;
 };
 
-// launchers (sc_kernels.hip)
+// launchers (sc_kernels.hip); every tile kernel is followed by the
+// one-block-per-replica k_finish that reduces its tile partials
 namespace sc {
-int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
-int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int mode, int sysmask, hipStream_t s);
-int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
-int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s,
+                 bool finish = true);
+int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s);
+int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s);
+int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
-int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
-int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
-int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
-int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int apply_rescale, hipStream_t s);
+int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int apply_rescale, hipStream_t s);
 int launch_from_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
 int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
 int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, int *counts, int cap, hipStream_t s);
-int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, double dt, double max_step, hipStream_t s);
-int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, double dt, hipStream_t s);
+int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, double max_step,
+                     hipStream_t s);
+int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, hipStream_t s);
 }  // namespace sc
 
 // fields for the host-order transfer kernels

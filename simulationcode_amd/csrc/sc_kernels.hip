@@ -7,10 +7,13 @@
 //
 // Execution model: one thread per particle slot, SC_TB (=256) slots per
 // tile, one block per tile (grid = tiles of all replicas in the launch).
-// Per-replica reductions (COM velocity, mean force, max displacement,
-// energies) are deterministic: tile partials in a fixed order, finished by
-// the last tile to arrive (atomic counter), which also resets the counter so
-// the launch sequence can be replayed (hipGraph) without host work.
+// No block ever waits for, or reads the results of, another block of the
+// same launch: on gfx950 an agent-scope release fence is an L2 write-back
+// (buffer_wbl2), far too costly per block.  Per-replica reductions (COM
+// velocity, mean force, max displacement, energies) are written as tile
+// partials and finished by k_finish (one block per replica) at the next
+// kernel boundary, in a fixed order, so they are deterministic and the
+// launch sequence can be replayed without host work.
 #include "sc_internal.h"
 
 #include <math.h>
@@ -85,40 +88,12 @@ __device__ inline void neumaier_add(double &s, double &c, double x)
     s = t;
 }
 
-// Last-tile-done: returns true in every thread of the tile that arrives last
-// for this replica's counter k; that tile must reset the counter.
-__device__ inline bool arrive_last(RepDyn *dyn, int k, int ntiles)
-{
-    __shared__ int last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned prev = atomicAdd(&dyn->ctr[k], 1u);
-        last = (prev == (unsigned)(ntiles - 1));
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last != 0;
-}
+// Per-tile partial slots (8 doubles per tile in Rep::part).
+enum { PS_VEC = 0, PS_MAXD2 = 3, PS_AUXMAX = 4, PS_ESUM = 5, PS_ECOMP = 6, PS_SCAL = 7 };
 
-// Sum the per-tile partials (K values each, stride 8) in a fixed order.
-template <int K>
-__device__ inline void reduce_partials(const Rep &R, double (&out)[K])
+__device__ inline void put_partial(const Rep &R, const Tile &t, int slot, double x)
 {
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) acc[k] = 0.0;
-    for (int t = threadIdx.x; t < R.ntiles; t += SC_TB)
-#pragma unroll
-        for (int k = 0; k < K; k++) acc[k] += __builtin_nontemporal_load(&R.part[t * 8 + k]);
-    block_sum<K>(acc);
-#pragma unroll
-    for (int k = 0; k < K; k++) out[k] = acc[k];
-}
-
-__device__ inline void atomic_max_nonneg(unsigned long long *p, double x)
-{
-    atomicMax(p, (unsigned long long)__double_as_longlong(x));
+    if (threadIdx.x == 0) R.part[t.local * 8 + slot] = x;
 }
 
 // ------------------------------------------------------------ Philox4x32-10
@@ -204,34 +179,42 @@ __device__ inline double pair_fdivr64(const Rep &R, double rij2, double Di, doub
     }
 }
 
-// FP32 force/r (mixed precision): same formulas, FP32 arithmetic with
-// hardware reciprocals.
+// FP32 force/r (mixed precision): same formulas, FP32 arithmetic; the
+// hardware reciprocal is refined by one Newton step (error ~1 ulp).
+__device__ inline float rcp32(float x)
+{
+    float r = __builtin_amdgcn_rcpf(x);
+    float e = fmaf(-x, r, 1.0f);
+    return fmaf(r, e, r);
+}
+
 template <int DIMS, int POT>
 __device__ inline float pair_fdivr32(const Rep &R, float rij2, float Di, float Dj)
 {
     if (POT == 2) {
         float md = (Di + Dj) * 0.5f * (1.0f - (float)R.zeta * fabsf(Di - Dj));
         float md2 = md * md;
-        float inv_md2 = __builtin_amdgcn_rcpf(md2);
+        float inv_md2 = rcp32(md2);
         float invxi2 = rij2 * inv_md2;
         float invxi4 = invxi2 * invxi2;
-        float xi14 = __builtin_amdgcn_rcpf(invxi4 * invxi4 * invxi4 * invxi2);
-        return ((float)R.fc2 * md2 < rij2) ? 0.0f
-               : -2.0f * inv_md2 * ((float)R.c2 + 2.0f * (float)R.c4 * invxi2 - 6.0f * xi14);
+        float xi14 = rcp32(invxi4 * invxi4 * invxi4 * invxi2);
+        float br = fmaf(2.0f * (float)R.c4, invxi2, (float)R.c2);
+        br = fmaf(-6.0f, xi14, br);
+        return ((float)R.fc2 * md2 < rij2) ? 0.0f : -2.0f * inv_md2 * br;
     } else if (DIMS == 3) {
-        float inv_r2 = __builtin_amdgcn_rcpf(rij2);
+        float inv_r2 = rcp32(rij2);
         float xi2 = (float)R.lj_s2 * inv_r2;
         float xi6 = xi2 * xi2 * xi2;
         float xi12 = xi6 * xi6;
-        return ((float)R.fc2 < rij2) ? 0.0f : (float)R.eps * (48.0f * xi12 - 24.0f * xi6) * inv_r2;
+        return ((float)R.fc2 < rij2) ? 0.0f : (float)R.eps * fmaf(48.0f, xi12, -24.0f * xi6) * inv_r2;
     } else {
         float md2 = (float)(R.lj_md * R.lj_md);
         if ((float)R.fc2 * md2 > rij2) {
-            float inv_r2 = __builtin_amdgcn_rcpf(rij2);
+            float inv_r2 = rcp32(rij2);
             float xi2 = md2 * inv_r2;
             float xi6 = xi2 * xi2 * xi2;
             float xi12 = xi6 * xi6;
-            return (float)R.eps * (48.0f * xi12 - 24.0f * xi6) * inv_r2;
+            return (float)R.eps * fmaf(48.0f, xi12, -24.0f * xi6) * inv_r2;
         }
         return 0.0f;
     }
@@ -370,17 +353,7 @@ __device__ void kick_drift_tile(const Rep &R, const Tile &t)
         s2 = s;
         if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
     }
-    double m = block_max(s2);
-    if (threadIdx.x == 0) atomic_max_nonneg(&dyn->maxdisp_bits, m);
-    if (arrive_last(dyn, CTR_DRIFT, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            double mx = __longlong_as_double((long long)atomicAdd(&dyn->maxdisp_bits, 0ull));
-            if (mx > R.thr) dyn->rebuild = 1;
-            dyn->maxdisp_bits = 0ull;
-            dyn->pending = 0;
-            dyn->ctr[CTR_DRIFT] = 0u;
-        }
-    }
+    put_partial(R, t, PS_MAXD2, block_max(s2));
 }
 
 __global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const Rep *reps)
@@ -426,16 +399,7 @@ __device__ void drift_brownian_tile(const Rep &R, const Tile &t)
         s2 = s;
         if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
     }
-    double m = block_max(s2);
-    if (threadIdx.x == 0) atomic_max_nonneg(&dyn->maxdisp_bits, m);
-    if (arrive_last(dyn, CTR_DRIFT, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            double mx = __longlong_as_double((long long)atomicAdd(&dyn->maxdisp_bits, 0ull));
-            if (mx > R.thr) dyn->rebuild = 1;
-            dyn->maxdisp_bits = 0ull;
-            dyn->ctr[CTR_DRIFT] = 0u;
-        }
-    }
+    put_partial(R, t, PS_MAXD2, block_max(s2));
 }
 
 __global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, const Rep *reps)
@@ -480,24 +444,23 @@ __device__ void force_tile(const Rep &R, const Tile &t)
                 if (DIMS == 3) f[2] += fs * dz;
             }
         } else {
-            float g[3] = {0.0f, 0.0f, 0.0f};
+            // FP64 minimum image; FP32 r^2 and force law; the FP32 force/r
+            // multiplies the FP64 separation and accumulates in FP64.
             const float Di = (float)pi.w;
             for (int k = 0; k < cnt; k++) {
                 const int j = nb[(size_t)k * n];
                 const double4 pj = R.pos[j];
-                float dx = (float)min_image(pi.x - pj.x, R.L, R.invL);
-                float dy = (float)min_image(pi.y - pj.y, R.L, R.invL);
-                float dz = DIMS == 3 ? (float)min_image(pi.z - pj.z, R.L, R.invL) : 0.0f;
-                float rij2 = dx * dx + dy * dy;
-                if (DIMS == 3) rij2 = rij2 + dz * dz;
-                float fs = pair_fdivr32<DIMS, POT>(R, rij2, Di, (float)pj.w);
-                g[0] += fs * dx;
-                g[1] += fs * dy;
-                if (DIMS == 3) g[2] += fs * dz;
+                const double dx = min_image(pi.x - pj.x, R.L, R.invL);
+                const double dy = min_image(pi.y - pj.y, R.L, R.invL);
+                const double dz = DIMS == 3 ? min_image(pi.z - pj.z, R.L, R.invL) : 0.0;
+                const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                float rij2 = fmaf(fy, fy, fx * fx);
+                if (DIMS == 3) rij2 = fmaf(fz, fz, rij2);
+                const double fs = (double)pair_fdivr32<DIMS, POT>(R, rij2, Di, (float)pj.w);
+                f[0] += fs * dx;
+                f[1] += fs * dy;
+                if (DIMS == 3) f[2] += fs * dz;
             }
-            f[0] = g[0];
-            f[1] = g[1];
-            f[2] = g[2];
         }
         // add_new_random_force! (src/MD.jl:30-43 Langevin, :67-78 Brownian)
         if (R.sys != 0) {
@@ -542,27 +505,7 @@ __device__ void force_tile(const Rep &R, const Tile &t)
         block_sum<3>(red);
         if (threadIdx.x == 0)
 #pragma unroll
-            for (int c = 0; c < 3; c++) R.part[t.local * 8 + c] = red[c];
-    }
-    if (arrive_last(dyn, CTR_FORCE, R.ntiles)) {
-        if (MODE != 0) {
-            double tot[3];
-            reduce_partials<3>(R, tot);
-            if (threadIdx.x == 0) {
-                for (int c = 0; c < DIMS; c++) {
-                    double mean = tot[c] / (double)n;
-                    if (MODE == 1)
-                        dyn->vmean[c] = mean;
-                    else
-                        dyn->Fmean[c] = mean;
-                }
-                if (MODE == 1) dyn->pending = 1 | (R.rescale ? 2 : 0);
-            }
-        }
-        if (threadIdx.x == 0) {
-            dyn->step = step + 1;
-            dyn->ctr[CTR_FORCE] = 0u;
-        }
+            for (int c = 0; c < 3; c++) R.part[t.local * 8 + PS_VEC + c] = red[c];
     }
 }
 
@@ -603,19 +546,7 @@ __device__ void ekin_com_tile(const Rep &R, const Tile &t)
         e[0] = s;
     }
     block_sum<1>(e);
-    if (threadIdx.x == 0) R.part[t.local * 8 + 4] = e[0];
-    if (arrive_last(dyn, CTR_EKCOM, R.ntiles)) {
-        double acc = 0.0;
-        for (int q = threadIdx.x; q < R.ntiles; q += SC_TB) acc += __builtin_nontemporal_load(&R.part[q * 8 + 4]);
-        double a[1] = {acc};
-        block_sum<1>(a);
-        if (threadIdx.x == 0) {
-            double ek = a[0] * (R.m / 2.0);
-            double kT = 2.0 * ek / ((double)DIMS * (double)n);
-            dyn->vscale = sqrt(R.kBT / kT);
-            dyn->ctr[CTR_EKCOM] = 0u;
-        }
-    }
+    put_partial(R, t, PS_SCAL, e[0]);
 }
 
 __global__ __launch_bounds__(SC_TB) void k_ekin_com(const Tile *tiles, const Rep *reps)
@@ -644,12 +575,6 @@ __global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep
             if (pend & 1) vc = vc - dyn->vmean[c];
             if (pend & 2) vc = vc * dyn->vscale;
             R.v[c * n + i] = vc;
-        }
-    }
-    if (arrive_last(dyn, CTR_MISC, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            dyn->pending = 0;
-            dyn->ctr[CTR_MISC] = 0u;
         }
     }
 }
@@ -828,15 +753,6 @@ __global__ __launch_bounds__(SC_TB) void k_build(const Tile *tiles, const Rep *r
         build_tile<3>(R, t);
     else
         build_tile<2>(R, t);
-    if (arrive_last(dyn, CTR_BUILD, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            dyn->rebuild = 0;
-            dyn->rebuilds += 1;
-            dyn->have_lists = 1;
-            dyn->maxdisp_bits = 0ull;
-            dyn->ctr[CTR_BUILD] = 0u;
-        }
-    }
 }
 
 // Full rows at the current positions over the rebuild-time cells, for the
@@ -922,20 +838,8 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
     if (threadIdx.x == 0) {
         double sum = 0.0, comp = 0.0;
         for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
-        R.part[t.local * 8 + 5] = sum;
-        R.part[t.local * 8 + 6] = comp;
-    }
-    __syncthreads();
-    if (arrive_last(dyn, CTR_ENERGY, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            double sum = 0.0, comp = 0.0;
-            for (int q = 0; q < R.ntiles; q++) {
-                neumaier_add(sum, comp, __builtin_nontemporal_load(&R.part[q * 8 + 5]));
-                comp += __builtin_nontemporal_load(&R.part[q * 8 + 6]);
-            }
-            dyn->result[0] = (sum + comp) / 2.0;
-            dyn->ctr[CTR_ENERGY] = 0u;
-        }
+        R.part[t.local * 8 + PS_ESUM] = sum;
+        R.part[t.local * 8 + PS_ECOMP] = comp;
     }
 }
 
@@ -981,20 +885,8 @@ __device__ void energy_nn_tile(const Rep &R, const Tile &t)
     if (threadIdx.x == 0) {
         double sum = 0.0, comp = 0.0;
         for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
-        R.part[t.local * 8 + 5] = sum;
-        R.part[t.local * 8 + 6] = comp;
-    }
-    __syncthreads();
-    if (arrive_last(dyn, CTR_ENERGY, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            double sum = 0.0, comp = 0.0;
-            for (int q = 0; q < R.ntiles; q++) {
-                neumaier_add(sum, comp, __builtin_nontemporal_load(&R.part[q * 8 + 5]));
-                comp += __builtin_nontemporal_load(&R.part[q * 8 + 6]);
-            }
-            dyn->result[0] = (sum + comp) / 2.0;
-            dyn->ctr[CTR_ENERGY] = 0u;
-        }
+        R.part[t.local * 8 + PS_ESUM] = sum;
+        R.part[t.local * 8 + PS_ECOMP] = comp;
     }
 }
 
@@ -1013,7 +905,7 @@ __global__ __launch_bounds__(SC_TB) void k_energy_nn(const Tile *tiles, const Re
 
 // find_kinetic_energy (src/AuxiliaryFunctions.jl:57-65); with apply != 0 it
 // also stages rescale_velocities! (src/MD.jl:391-402) as a pending scale.
-__global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *reps, int apply)
+__global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *reps)
 {
     const Tile t = tiles[blockIdx.x];
     const Rep &R = reps[t.rep];
@@ -1029,23 +921,7 @@ __global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *re
         e[0] = s;
     }
     block_sum<1>(e);
-    if (threadIdx.x == 0) R.part[t.local * 8 + 7] = e[0];
-    if (arrive_last(dyn, CTR_EKIN, R.ntiles)) {
-        double acc = 0.0;
-        for (int q = threadIdx.x; q < R.ntiles; q += SC_TB) acc += __builtin_nontemporal_load(&R.part[q * 8 + 7]);
-        double a[1] = {acc};
-        block_sum<1>(a);
-        if (threadIdx.x == 0) {
-            double ek = a[0] * (R.m / 2.0);
-            dyn->result[1] = ek;
-            if (apply) {
-                double kT = 2.0 * ek / ((double)R.dims * (double)n);
-                dyn->vscale = sqrt(R.kBT / kT);
-                dyn->pending = 2;
-            }
-            dyn->ctr[CTR_EKIN] = 0u;
-        }
-    }
+    put_partial(R, t, PS_SCAL, e[0]);
 }
 
 // ====================================================================
@@ -1118,7 +994,7 @@ __global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Re
 // Capped steepest descent (find_inherent_structure,
 // src/InherentStructure.jl:26-51)
 // ====================================================================
-__global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Rep *reps, double dt, double max_step)
+__global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Rep *reps, double dt)
 {
     const Tile t = tiles[blockIdx.x];
     const Rep &R = reps[t.rep];
@@ -1138,22 +1014,8 @@ __global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Re
     }
     double m1 = block_max(st);
     double m2 = block_max(fm);
-    if (threadIdx.x == 0) {
-        atomic_max_nonneg(&dyn->maxaux_bits, m1);
-        R.part[t.local * 8 + 3] = m2;
-    }
-    if (arrive_last(dyn, CTR_MISC, R.ntiles)) {
-        double mf = 0.0;
-        for (int q = threadIdx.x; q < R.ntiles; q += SC_TB) mf = fmax(mf, __builtin_nontemporal_load(&R.part[q * 8 + 3]));
-        mf = block_max(mf);
-        if (threadIdx.x == 0) {
-            double stepsize = __longlong_as_double((long long)atomicAdd(&dyn->maxaux_bits, 0ull));
-            dyn->result[2] = (stepsize > max_step) ? max_step / stepsize : -1.0;  // -1: unscaled
-            dyn->result[3] = mf;
-            dyn->maxaux_bits = 0ull;
-            dyn->ctr[CTR_MISC] = 0u;
-        }
-    }
+    put_partial(R, t, PS_AUXMAX, m1);
+    put_partial(R, t, PS_VEC, m2);
 }
 
 template <int DIMS>
@@ -1185,16 +1047,7 @@ __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
         s2 = s;
         if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
     }
-    double m = block_max(s2);
-    if (threadIdx.x == 0) atomic_max_nonneg(&dyn->maxdisp_bits, m);
-    if (arrive_last(dyn, CTR_DRIFT, R.ntiles)) {
-        if (threadIdx.x == 0) {
-            double mx = __longlong_as_double((long long)atomicAdd(&dyn->maxdisp_bits, 0ull));
-            if (mx > R.thr) dyn->rebuild = 1;
-            dyn->maxdisp_bits = 0ull;
-            dyn->ctr[CTR_DRIFT] = 0u;
-        }
-    }
+    put_partial(R, t, PS_MAXD2, block_max(s2));
 }
 
 __global__ __launch_bounds__(SC_TB) void k_min_move(const Tile *tiles, const Rep *reps, double dt)
@@ -1207,46 +1060,198 @@ __global__ __launch_bounds__(SC_TB) void k_min_move(const Tile *tiles, const Rep
         min_move_tile<2>(R, t, dt);
 }
 
+// ====================================================================
+// k_finish: one block per replica turns the tile partials of the previous
+// launch into per-replica results (fixed order => deterministic).
+// ====================================================================
+enum {
+    FIN_DRIFT = 0,      // max displacement^2 -> rebuild flag; pending COM consumed
+    FIN_FORCE0,         // force evaluation counter
+    FIN_FORCE1,         // + COM velocity (pending)
+    FIN_FORCE2,         // + mean force (Brownian)
+    FIN_EKCOM,          // rescale factor from the COM-shifted kinetic energy
+    FIN_FINALIZE,       // pending COM applied
+    FIN_REBUILD_DONE,   // rebuild bookkeeping
+    FIN_ENERGY,         // compensated energy total, halved
+    FIN_EKIN,           // kinetic energy
+    FIN_EKIN_RESCALE,   // + rescale_velocities! factor (pending)
+    FIN_MINSCALE        // minimiser step scale and max |F|
+};
+
+__global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, int op, int sysmask, double arg)
+{
+    const int r = blockIdx.x;
+    if (r >= nreps) return;
+    const Rep &R = reps[r];
+    if (!((sysmask >> R.sys) & 1)) return;
+    RepDyn *dyn = R.dyn;
+    const int tid = threadIdx.x;
+    const double *P = R.part;
+    switch (op) {
+    case FIN_DRIFT: {
+        double m = 0.0;
+        for (int q = tid; q < R.ntiles; q += SC_TB) m = fmax(m, P[q * 8 + PS_MAXD2]);
+        m = block_max(m);
+        if (tid == 0) {
+            if (m > R.thr) dyn->rebuild = 1;
+            dyn->pending = 0;
+        }
+    } break;
+    case FIN_FORCE0:
+        if (tid == 0) dyn->step += 1;
+        break;
+    case FIN_FORCE1:
+    case FIN_FORCE2: {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (int q = tid; q < R.ntiles; q += SC_TB)
+            for (int c = 0; c < 3; c++) a[c] += P[q * 8 + PS_VEC + c];
+        block_sum<3>(a);
+        if (tid == 0) {
+            for (int c = 0; c < R.dims; c++) {
+                double mean = a[c] / (double)R.n;
+                if (op == FIN_FORCE1)
+                    dyn->vmean[c] = mean;
+                else
+                    dyn->Fmean[c] = mean;
+            }
+            if (op == FIN_FORCE1) dyn->pending = 1 | (R.rescale ? 2 : 0);
+            dyn->step += 1;
+        }
+    } break;
+    case FIN_EKCOM:
+    case FIN_EKIN:
+    case FIN_EKIN_RESCALE: {
+        double a[1] = {0.0};
+        for (int q = tid; q < R.ntiles; q += SC_TB) a[0] += P[q * 8 + PS_SCAL];
+        block_sum<1>(a);
+        if (tid == 0) {
+            double ek = a[0] * (R.m / 2.0);
+            double kT = 2.0 * ek / ((double)R.dims * (double)R.n);
+            if (op == FIN_EKCOM) {
+                dyn->vscale = sqrt(R.kBT / kT);
+            } else {
+                dyn->result[1] = ek;
+                if (op == FIN_EKIN_RESCALE) {
+                    dyn->vscale = sqrt(R.kBT / kT);
+                    dyn->pending = 2;
+                }
+            }
+        }
+    } break;
+    case FIN_FINALIZE:
+        if (tid == 0) dyn->pending = 0;
+        break;
+    case FIN_REBUILD_DONE:
+        if (tid == 0 && dyn->rebuild) {
+            dyn->rebuild = 0;
+            dyn->rebuilds += 1;
+            dyn->have_lists = 1;
+        }
+        break;
+    case FIN_ENERGY:
+        if (tid == 0) {
+            double sum = 0.0, comp = 0.0;
+            for (int q = 0; q < R.ntiles; q++) {
+                neumaier_add(sum, comp, P[q * 8 + PS_ESUM]);
+                comp += P[q * 8 + PS_ECOMP];
+            }
+            dyn->result[0] = (sum + comp) / 2.0;
+        }
+        break;
+    case FIN_MINSCALE: {
+        double st = 0.0, fm = 0.0;
+        for (int q = tid; q < R.ntiles; q += SC_TB) {
+            st = fmax(st, P[q * 8 + PS_AUXMAX]);
+            fm = fmax(fm, P[q * 8 + PS_VEC]);
+        }
+        st = block_max(st);
+        fm = block_max(fm);
+        if (tid == 0) {
+            dyn->result[2] = (st > arg) ? arg / st : -1.0;  // -1: unscaled
+            dyn->result[3] = fm;
+        }
+    } break;
+    }
+}
+
 inline int check(hipError_t e) { return e == hipSuccess ? 0 : -5; }
 
 }  // namespace
 
 // ====================================================================
-// Launchers
+// Launchers: each tile kernel is followed by its k_finish.
 // ====================================================================
 namespace sc {
 
-#define LAUNCH(kern, grid, block, s, ...)                                 \
-    do {                                                                 \
-        if ((grid) > 0) hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, s, __VA_ARGS__); \
-        return check(hipGetLastError());                                 \
-    } while (0)
+constexpr int SYS_ALL = 7, SYS_VV = 3, SYS_BD = 4, SYS_NEWT = 1;
+int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s);
 
-int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+static inline void fin(const Rep *reps, int nreps, int op, int mask, double arg, hipStream_t s)
 {
-    LAUNCH(k_kick_drift, ntiles, SC_TB, s, tiles, reps);
+    hipLaunchKernelGGL(k_finish, dim3(nreps), dim3(SC_TB), 0, s, reps, nreps, op, mask, arg);
 }
 
-int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int mode, int sysmask, hipStream_t s)
+int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
-    if (mode == 0) LAUNCH(k_force<0>, ntiles, SC_TB, s, tiles, reps, sysmask);
-    if (mode == 1) LAUNCH(k_force<1>, ntiles, SC_TB, s, tiles, reps, sysmask);
-    LAUNCH(k_force<2>, ntiles, SC_TB, s, tiles, reps, sysmask);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_DRIFT, SYS_VV, 0.0, s);
+    return check(hipGetLastError());
 }
 
-int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s,
+                 bool finish)
 {
-    LAUNCH(k_drift_brownian, ntiles, SC_TB, s, tiles, reps);
+    if (ntiles <= 0) return 0;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_force<0>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_force<1>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask);
+    else
+        hipLaunchKernelGGL(k_force<2>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask);
+    if (finish) return launch_force_finish(reps, nreps, mode, sysmask, s);
+    return check(hipGetLastError());
 }
 
-int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s)
 {
-    LAUNCH(k_ekin_com, ntiles, SC_TB, s, tiles, reps);
+    fin(reps, nreps, mode == 0 ? FIN_FORCE0 : (mode == 1 ? FIN_FORCE1 : FIN_FORCE2), sysmask, 0.0, s);
+    return check(hipGetLastError());
+}
+
+int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    return check(hipGetLastError());
+}
+
+int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s)
+{
+    fin(reps, nreps, FIN_DRIFT, sysmask, 0.0, s);
+    return check(hipGetLastError());
+}
+
+int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_drift_brownian, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_DRIFT, SYS_BD, 0.0, s);
+    return check(hipGetLastError());
+}
+
+int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_ekin_com, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_EKCOM, SYS_NEWT, 0.0, s);
+    return check(hipGetLastError());
 }
 
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s)
 {
-    LAUNCH(k_request_rebuild, (nreps + 255) / 256, 256, s, const_cast<Rep *>(reps), nreps);
+    hipLaunchKernelGGL(k_request_rebuild, dim3((nreps + 255) / 256), dim3(256), 0, s, const_cast<Rep *>(reps), nreps);
+    return check(hipGetLastError());
 }
 
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
@@ -1258,52 +1263,78 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hi
     hipLaunchKernelGGL(k_cellsort, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
     hipLaunchKernelGGL(k_permute, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
     hipLaunchKernelGGL(k_build, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }
 
-int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
-    LAUNCH(k_finalize, ntiles, SC_TB, s, tiles, reps);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_finalize, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_FINALIZE, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
 }
 
-int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
-    LAUNCH(k_energy, ntiles, SC_TB, s, tiles, reps);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_energy, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_ENERGY, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
 }
 
-int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
-    LAUNCH(k_energy_nn, ntiles, SC_TB, s, tiles, reps);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_energy_nn, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, FIN_ENERGY, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
 }
 
-int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int apply_rescale, hipStream_t s)
+int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int apply_rescale, hipStream_t s)
 {
-    LAUNCH(k_ekin, ntiles, SC_TB, s, tiles, reps, apply_rescale);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_ekin, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    fin(reps, nreps, apply_rescale ? FIN_EKIN_RESCALE : FIN_EKIN, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
 }
 
 int launch_from_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s)
 {
-    LAUNCH(k_from_host, ntiles, SC_TB, s, tiles, reps, field);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_from_host, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, field);
+    return check(hipGetLastError());
 }
 
 int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s)
 {
-    LAUNCH(k_to_host, ntiles, SC_TB, s, tiles, reps, field);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_to_host, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, field);
+    return check(hipGetLastError());
 }
 
 int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, int *counts, int cap, hipStream_t s)
 {
-    LAUNCH(k_full_rows, ntiles, SC_TB, s, tiles, reps, rows, counts, cap);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_full_rows, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, rows, counts, cap);
+    return check(hipGetLastError());
 }
 
-int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, double dt, double max_step, hipStream_t s)
+int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, double max_step,
+                     hipStream_t s)
 {
-    LAUNCH(k_min_scale, ntiles, SC_TB, s, tiles, reps, dt, max_step);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_min_scale, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, dt);
+    fin(reps, nreps, FIN_MINSCALE, SYS_ALL, max_step, s);
+    return check(hipGetLastError());
 }
 
-int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, double dt, hipStream_t s)
+int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, hipStream_t s)
 {
-    LAUNCH(k_min_move, ntiles, SC_TB, s, tiles, reps, dt);
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_min_move, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, dt);
+    fin(reps, nreps, FIN_DRIFT, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
 }
 
 void philox_host(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]) { philox4x32_10(in, key, out); }
