@@ -41,12 +41,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1):
+def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1, cell_divisor=1):
     from simulationcode_amd import md
     U = md.Berthier(rc=1.25)
     p = nat.ScParams()
     p.dims, p.potential, p.system, p.precision = 3, nat.POT_BERTHIER, nat.SYS_NEWTONIAN, precision
-    p.rescale_temperature, p.cell_divisor = 0, 1
+    p.rescale_temperature, p.cell_divisor = 0, cell_divisor
     p.N, p.maxneighbours = N, 200
     p.box_size, p.force_cutoff, p.force_cutoff2, p.skin_distance = L, 1.25, 1.25 ** 2, 0.3
     p.b_c0, p.b_c2, p.b_c4, p.b_zeta = U.c0, U.c2, U.c4, U.ζ
@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--seed", type=int, default=20261206)
+    ap.add_argument("--cell-divisor", type=int, default=0, help="0: library default")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -120,7 +121,7 @@ def main():
     r = synth.lattice_positions(N, dims, L, 0.1, rng)
 
     log(f"[bench] setup N={N} L={L:.3f}")
-    ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64)
+    ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64, cell_divisor=args.cell_divisor)
     ctx.set_state(r=r, v=np.zeros((N, dims)), F=np.zeros((N, dims)))
     ctx.update_neighbor_lists()
     t0 = time.perf_counter()
@@ -168,7 +169,7 @@ def main():
                     "wall_s": res64["wall_s"]}}
 
     if not args.no_mixed:
-        ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED)
+        ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED, cell_divisor=args.cell_divisor)
         ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
         ctxm.update_neighbor_lists()
         ctx.set_state(r=state["r"], v=state["v"], F=state["F"])

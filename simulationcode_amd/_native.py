@@ -68,6 +68,8 @@ class ScParams(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("rescale_temperature", ctypes.c_int32),
         ("cell_divisor", ctypes.c_int32),
+        ("build_mode", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("N", ctypes.c_int64),
         ("maxneighbours", ctypes.c_int64),
         ("box_size", ctypes.c_double),

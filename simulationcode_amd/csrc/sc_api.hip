@@ -34,6 +34,7 @@ struct sc_ctx {
     std::vector<void *> allocs;
     bool timing = false;
     std::vector<hipEvent_t> kev;
+    sc::RebuildGeom geom{};
     double t_force = 0.0, t_drift = 0.0;
     long long n_force = 0, n_drift = 0;
 };
@@ -196,6 +197,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     h.skin = p->skin_distance;
     h.Rv2 = Rv * Rv;
     h.thr = (p->skin_distance / 2.0) * (p->skin_distance / 2.0);
+    std::memcpy(&h.thr_bits, &h.thr, sizeof(double));
     h.eps = p->lj_eps;
     h.lj_s2 = p->lj_sigma * p->lj_sigma;
     h.lj_md = p->lj_sigma * p->lj_sigma;  // find_mean_D for LJ returns sigma^2
@@ -211,7 +213,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     h.seed = p->seed;
     // cells of side >= R_v / k with a (2k+1)^dims stencil; a hair of slack so
     // floating-point cell assignment can never drop a pair at exactly R_v
-    const int k = p->cell_divisor > 0 ? p->cell_divisor : 1;
+    const int k = p->cell_divisor > 0 ? p->cell_divisor : 2;
     long long nc = (long long)std::floor(k * p->box_size / (Rv * (1.0 + 1e-10)));
     int srange = k;
     if (nc < 2 * k + 1) {
@@ -231,6 +233,24 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     c->ntiles = (n + SC_TB - 1) / SC_TB;
     h.ntiles = c->ntiles;
     h.tile0 = 0;
+    // LDS build runs: ~64 own particles per block, window <= 64 cells
+    h.lds_build = (srange == 1 && p->build_mode == 1) ? 1 : 0;
+    if (h.lds_build) {
+        const double per_cell = (double)n / (double)ncell;
+        const int cxmax = (p->dims == 3 ? 5 : 19);
+        int cx = (int)std::lround(64.0 / std::max(per_cell, 1e-9));
+        cx = std::max(1, std::min(cx, std::min(cxmax, (int)nc - 2)));
+        h.cx = cx;
+        const long long rpr = (nc + cx - 1) / cx;
+        h.nruns = (int)(rpr * nc * (p->dims == 3 ? nc : 1));
+    } else {
+        h.cx = 0;
+        h.nruns = 0;
+    }
+    h.run0 = 0;
+    c->geom.max_chunks = (int)((ncell + SC_CHUNK - 1) / SC_CHUNK);
+    c->geom.nruns = h.nruns;
+    c->geom.any_tile_build = h.lds_build ? 0 : 1;
     *out = c;
 
     if (hipStreamCreate(&c->s) != hipSuccess) return fail(c, SC_ERR_HIP, "hipStreamCreate failed");
@@ -248,7 +268,8 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     rc |= dalloc(c, &h.orig_alt, n);
     rc |= dalloc(c, &h.cell_of, n);
     rc |= dalloc(c, &h.rank, n);
-    rc |= dalloc(c, &h.perm, n);
+    rc |= dalloc(c, &h.key, n);
+    rc |= dalloc(c, &h.chunk_off, (size_t)c->geom.max_chunks);
     rc |= dalloc(c, &h.cell_cnt, (size_t)ncell);
     rc |= dalloc(c, &h.cell_start, (size_t)ncell + 1);
     rc |= dalloc(c, &h.nn, n);
@@ -278,6 +299,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     HIPCHK(c, hipMemset(h.cell_cnt, 0, sizeof(int) * ncell));
     HIPCHK(c, hipMemset(h.nn, 0, sizeof(int) * n));
     HIPCHK(c, hipMemset(h.part, 0, sizeof(double) * c->ntiles * 8));
+    HIPCHK(c, hipMemset(h.chunk_off, 0, sizeof(int) * c->geom.max_chunks));
     HIPCHK(c, hipMemcpy(h.dyn, &dyn, sizeof(RepDyn), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_rep, &h, sizeof(Rep), hipMemcpyHostToDevice));
     return SC_OK;
@@ -328,7 +350,7 @@ int sc_update_neighbor_lists(sc_ctx *c)
 {
     if (!c || !c->d_rep) return SC_ERR_ARG;
     KCHK(c, sc::launch_request_rebuild(c->d_rep, 1, c->s));
-    KCHK(c, sc::launch_rebuild(c->d_tiles, c->ntiles, c->d_rep, 1, c->s));
+    KCHK(c, sc::launch_rebuild(c->d_tiles, c->ntiles, c->d_rep, 1, c->geom, c->s));
     return check_dyn(c);
 }
 
@@ -421,13 +443,12 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, 1, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s));
         } else {
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_kick_drift_only(c->d_tiles, nt, c->d_rep, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_drift_finish(c->d_rep, 1, 3, c->s));
-            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s, false));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 1, 7, c->s, false));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
@@ -541,7 +562,7 @@ int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *ma
         KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 0, 7, c->s));
         KCHK(c, sc::launch_min_scale(c->d_tiles, nt, c->d_rep, 1, dt, max_step, c->s));
         KCHK(c, sc::launch_min_move(c->d_tiles, nt, c->d_rep, 1, dt, c->s));
-        KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->s));
+        KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s));
     }
     c->host_step += iters;
     c->h = saved;

@@ -19,6 +19,11 @@
 #include <stdint.h>
 
 #define SC_TB 256  // threads per tile / block (4 wavefronts)
+#define SC_CHUNK_SHIFT 10
+#define SC_CHUNK (1 << SC_CHUNK_SHIFT)  // cells per block of the two-level scan
+#define SC_WMAX 1024                    // particles of one LDS-staged build window
+#define SC_GRID_CAP 2048                // blocks of the grid-stride rebuild kernels
+#define SC_RUN_GRID_CAP 1024            // blocks of the LDS build
 
 // error bits raised on the device
 #define SC_EBIT_OVERFLOW 1
@@ -53,8 +58,12 @@ struct Rep {
     int cap;             // maxneighbours (row capacity)
     int rescale;
     int ntiles, tile0;   // tiles of this replica in the launch table
+    int lds_build;       // 1: LDS-staged window build (stencil +-1); 0: per-slot build
+    int cx;              // cells per LDS build run (along x)
+    int nruns, run0;     // LDS build runs of this replica and their offset in the launch
     double L, invL, cinv;          // box, 1/box, cells per unit length
     double fc2, rcut, skin, Rv2, thr;
+    unsigned long long thr_bits;  // bit pattern of thr (= (skin/2)^2)
     double eps, lj_s2, lj_md, Uc;  // LJ: sigma^2 (specialised loops), find_mean_D = sigma^2
     double c0, c2, c4, zeta;
     double kBT, gamma, dt, m;
@@ -66,9 +75,9 @@ struct Rep {
     double4 *pos, *pos_alt;
     double *v, *v_alt, *F, *disp;
     int *orig, *orig_alt;
-    int *cell_of, *rank, *cell_cnt, *cell_start, *perm;
+    int *cell_of, *rank, *cell_cnt, *cell_start, *chunk_off, *key;
     int *nn, *nbr;
-    double *part;        // per-tile partial sums, 8 doubles per tile
+    double *part;        // per-tile partials, slot-major: part[slot * ntiles + tile], 8 slots
     double *stage;       // n*3 doubles staging for host transfers
     RepDyn *dyn;
 };
@@ -91,7 +100,15 @@ int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipSt
 int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s);
 int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
-int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+struct RebuildGeom {
+    int max_chunks;      // max over replicas of ceil(ncell / SC_CHUNK)
+    int nruns;           // LDS build runs over all replicas
+    int any_tile_build;  // some replica uses the per-slot build (all-pairs / wide stencil)
+};
+// bookkeep=false when the next launch is a velocity-Verlet force finish,
+// which does the rebuild bookkeeping itself
+int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
+                   bool bookkeep = true);
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
 int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);

@@ -88,12 +88,39 @@ __device__ inline void neumaier_add(double &s, double &c, double x)
     s = t;
 }
 
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs (block b runs on XCD b % 8), each with its own L2.  Tiles are
+// cell-sorted, so spatial neighbours are nearby tiles: give every XCD one
+// contiguous run of tiles, swept in order, so neighbour gathers hit its L2.
+constexpr int NXCD = 8;
+
+__device__ inline int xcd_tile()
+{
+    const int b = blockIdx.x, nt = gridDim.x;
+    if (nt < NXCD) return b;
+    const int x = b % NXCD, k = b / NXCD;
+    const int base = nt / NXCD, rem = nt % NXCD;
+    return x * base + (x < rem ? x : rem) + k;
+}
+
+// The rebuild runs when the host requested it or the largest displacement^2
+// since the last rebuild exceeds (skin/2)^2 (src/MD.jl:523).  Only that
+// decision is needed, so a tile whose own maximum exceeds the threshold sets
+// the flag (max(...) > thr  <=>  some tile's max > thr); ordinary steps issue
+// no atomics at all.
+__device__ inline bool need_rebuild(const Rep &R) { return R.dyn->rebuild != 0; }
+
+__device__ inline void publish_maxdisp(const Rep &R, double m)
+{
+    if (threadIdx.x == 0 && m > R.thr) atomicOr(&R.dyn->rebuild, 1);
+}
+
 // Per-tile partial slots (8 doubles per tile in Rep::part).
 enum { PS_VEC = 0, PS_MAXD2 = 3, PS_AUXMAX = 4, PS_ESUM = 5, PS_ECOMP = 6, PS_SCAL = 7 };
 
 __device__ inline void put_partial(const Rep &R, const Tile &t, int slot, double x)
 {
-    if (threadIdx.x == 0) R.part[t.local * 8 + slot] = x;
+    if (threadIdx.x == 0) R.part[slot * R.ntiles + t.local] = x;
 }
 
 // ------------------------------------------------------------ Philox4x32-10
@@ -353,13 +380,13 @@ __device__ void kick_drift_tile(const Rep &R, const Tile &t)
         s2 = s;
         if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
     }
-    put_partial(R, t, PS_MAXD2, block_max(s2));
+    publish_maxdisp(R, block_max(s2));
 }
 
 __global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     if (R.sys == 2) return;
     if (R.dims == 3)
         kick_drift_tile<3>(R, t);
@@ -399,13 +426,13 @@ __device__ void drift_brownian_tile(const Rep &R, const Tile &t)
         s2 = s;
         if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
     }
-    put_partial(R, t, PS_MAXD2, block_max(s2));
+    publish_maxdisp(R, block_max(s2));
 }
 
 __global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     if (R.sys != 2) return;
     if (R.dims == 3)
         drift_brownian_tile<3>(R, t);
@@ -433,7 +460,27 @@ __device__ void force_tile(const Rep &R, const Tile &t)
         const int cnt = R.nn[i];
         const int *nb = R.nbr + i;
         if (PREC == 0) {
-            for (int k = 0; k < cnt; k++) {
+            // gathers issued four at a time so several are in flight per
+            // lane; accumulation order stays k = 0, 1, 2, ...
+            int k = 0;
+            for (; k + 4 <= cnt; k += 4) {
+                int jj[4];
+                double4 pp[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) jj[u] = nb[(size_t)(k + u) * n];
+#pragma unroll
+                for (int u = 0; u < 4; u++) pp[u] = R.pos[jj[u]];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    double dx, dy, dz;
+                    double rij2 = sqdist<DIMS>(R, pi, pp[u], dx, dy, dz);
+                    double fs = pair_fdivr64<DIMS, POT>(R, rij2, pi.w, pp[u].w);
+                    f[0] += fs * dx;
+                    f[1] += fs * dy;
+                    if (DIMS == 3) f[2] += fs * dz;
+                }
+            }
+            for (; k < cnt; k++) {
                 const int j = nb[(size_t)k * n];
                 const double4 pj = R.pos[j];
                 double dx, dy, dz;
@@ -447,7 +494,29 @@ __device__ void force_tile(const Rep &R, const Tile &t)
             // FP64 minimum image; FP32 r^2 and force law; the FP32 force/r
             // multiplies the FP64 separation and accumulates in FP64.
             const float Di = (float)pi.w;
-            for (int k = 0; k < cnt; k++) {
+            int k = 0;
+            for (; k + 4 <= cnt; k += 4) {
+                int jj[4];
+                double4 pp[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) jj[u] = nb[(size_t)(k + u) * n];
+#pragma unroll
+                for (int u = 0; u < 4; u++) pp[u] = R.pos[jj[u]];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const double dx = min_image(pi.x - pp[u].x, R.L, R.invL);
+                    const double dy = min_image(pi.y - pp[u].y, R.L, R.invL);
+                    const double dz = DIMS == 3 ? min_image(pi.z - pp[u].z, R.L, R.invL) : 0.0;
+                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                    float rij2 = fmaf(fy, fy, fx * fx);
+                    if (DIMS == 3) rij2 = fmaf(fz, fz, rij2);
+                    const double fs = (double)pair_fdivr32<DIMS, POT>(R, rij2, Di, (float)pp[u].w);
+                    f[0] += fs * dx;
+                    f[1] += fs * dy;
+                    if (DIMS == 3) f[2] += fs * dz;
+                }
+            }
+            for (; k < cnt; k++) {
                 const int j = nb[(size_t)k * n];
                 const double4 pj = R.pos[j];
                 const double dx = min_image(pi.x - pj.x, R.L, R.invL);
@@ -505,15 +574,15 @@ __device__ void force_tile(const Rep &R, const Tile &t)
         block_sum<3>(red);
         if (threadIdx.x == 0)
 #pragma unroll
-            for (int c = 0; c < 3; c++) R.part[t.local * 8 + PS_VEC + c] = red[c];
+            for (int c = 0; c < 3; c++) R.part[(PS_VEC + c) * R.ntiles + t.local] = red[c];
     }
 }
 
 template <int MODE>
 __global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *reps, int sysmask)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     if (!((sysmask >> R.sys) & 1)) return;
     const int key = (R.dims == 3 ? 4 : 0) + (R.pot == 2 ? 2 : 0) + (R.prec ? 1 : 0);
     switch (key) {
@@ -551,8 +620,8 @@ __device__ void ekin_com_tile(const Rep &R, const Tile &t)
 
 __global__ __launch_bounds__(SC_TB) void k_ekin_com(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     if (!R.rescale || R.sys != 0) return;
     if (R.dims == 3)
         ekin_com_tile<3>(R, t);
@@ -564,8 +633,8 @@ __global__ __launch_bounds__(SC_TB) void k_ekin_com(const Tile *tiles, const Rep
 // between API calls.
 __global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     RepDyn *dyn = R.dyn;
     const int pend = dyn->pending;
     const int n = R.n, i = t.first + threadIdx.x;
@@ -591,168 +660,470 @@ __global__ void k_request_rebuild(Rep *reps, int nreps)
     if (r < nreps) reps[r].dyn->rebuild = 1;
 }
 
-// R1: cell of every slot and its arrival rank within the cell.
-__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps)
+// Slot range start of cell c: chunk-local exclusive prefix + chunk offset
+// (two-level scan), c == ncell gives n.
+__device__ inline int cstart(const Rep &R, int c)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
-    if (!R.dyn->rebuild) return;
-    const int i = t.first + threadIdx.x;
-    if (i >= R.n) return;
-    double4 p = R.pos[i];
-    int cc[3];
-    if (R.dims == 3)
-        cell_of_pos<3>(R, p, cc);
-    else
-        cell_of_pos<2>(R, p, cc);
-    int c = cc[0] + R.nc * (cc[1] + R.nc * cc[2]);
-    R.cell_of[i] = c;
-    R.rank[i] = atomicAdd(&R.cell_cnt[c], 1);
+    return c >= R.ncell ? R.n : R.cell_start[c] + R.chunk_off[c >> SC_CHUNK_SHIFT];
 }
 
-// R2: exclusive scan of the cell counts (one block per replica); clears the
-// counts for the next rebuild.
-__global__ __launch_bounds__(1024) void k_scan(const Rep *reps, int nreps)
+// XCD-aware grid-stride iteration for capped grids: the blocks resident on
+// one XCD sweep one contiguous range of work items, in order.
+struct XcdRange {
+    int cur, end, step;
+};
+
+__device__ inline XcdRange xcd_range(int total)
+{
+    const int b = blockIdx.x, g = gridDim.x;
+    if (g < NXCD) return XcdRange{b, total, g};
+    const int x = b % NXCD, kb = b / NXCD;
+    const int nbx = (g - x + NXCD - 1) / NXCD;  // blocks on this XCD
+    const int base = total / NXCD, rem = total % NXCD;
+    const int c0 = x * base + (x < rem ? x : rem);
+    const int c1 = c0 + base + (x < rem ? 1 : 0);
+    return XcdRange{c0 + kb, c1, nbx};
+}
+
+// R1: cell of every slot and its arrival rank within the cell.  Slots are
+// cell-sorted from the previous rebuild, so a wavefront spans few cells:
+// one atomic per distinct cell per wavefront.
+__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, int ntiles)
+{
+    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[it.cur];
+        const Rep R = reps[t.rep];
+        if (!need_rebuild(R)) continue;
+        const int i = t.first + threadIdx.x;
+        const bool valid = i < R.n;
+        int c = -1;
+        if (valid) {
+            double4 p = R.pos[i];
+            int cc[3];
+            if (R.dims == 3)
+                cell_of_pos<3>(R, p, cc);
+            else
+                cell_of_pos<2>(R, p, cc);
+            c = cc[0] + R.nc * (cc[1] + R.nc * cc[2]);
+            R.cell_of[i] = c;
+        }
+        const int lane = threadIdx.x % WAVE;
+        unsigned long long active = __ballot(valid);
+        int rank = 0;
+        while (active) {
+            const int leader = __ffsll((long long)active) - 1;
+            const int lc = __shfl(c, leader, WAVE);
+            const unsigned long long same = __ballot(valid && c == lc) & active;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&R.cell_cnt[lc], __popcll(same));
+            base = __shfl(base, leader, WAVE);
+            if ((same >> lane) & 1ull) rank = base + __popcll(same & ((1ull << lane) - 1ull));
+            active &= ~same;
+        }
+        if (valid) R.rank[i] = rank;
+    }
+}
+
+// R2a: chunk-local exclusive scan of the cell counts (SC_CHUNK cells per
+// block, 4 per thread); clears the counts for the next rebuild.
+__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, int nreps)
+{
+    const int r = blockIdx.y;
+    if (r >= nreps) return;
+    const Rep R = reps[r];
+    if (!need_rebuild(R)) return;
+    const int c0 = blockIdx.x * SC_CHUNK;
+    if (c0 >= R.ncell) return;
+    const int tid = threadIdx.x;
+    int v[4];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = c0 + tid * 4 + q;
+        v[q] = c < R.ncell ? R.cell_cnt[c] : 0;
+        s += v[q];
+    }
+    __shared__ int sums[SC_TB];
+    sums[tid] = s;
+    __syncthreads();
+    for (int o = 1; o < SC_TB; o <<= 1) {
+        int x = tid >= o ? sums[tid - o] : 0;
+        __syncthreads();
+        sums[tid] += x;
+        __syncthreads();
+    }
+    int run = sums[tid] - s;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = c0 + tid * 4 + q;
+        if (c < R.ncell) {
+            R.cell_start[c] = run;
+            R.cell_cnt[c] = 0;
+        }
+        run += v[q];
+    }
+    if (tid == SC_TB - 1) R.chunk_off[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
+}
+
+// R2b: exclusive scan of the chunk totals (one block per replica).
+__global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, int nreps)
 {
     const int r = blockIdx.x;
     if (r >= nreps) return;
-    const Rep &R = reps[r];
-    if (!R.dyn->rebuild) return;
-    __shared__ int sums[1024];
-    const int nt = blockDim.x, tid = threadIdx.x;
-    const int nc = R.ncell;
-    const int chunk = (nc + nt - 1) / nt;
-    const int b = tid * chunk, e = min(nc, b + chunk);
-    int s = 0;
-    for (int c = b; c < e; c++) s += R.cell_cnt[c];
-    sums[tid] = s;
+    const Rep R = reps[r];
+    if (!need_rebuild(R)) return;
+    const int nch = (R.ncell + SC_CHUNK - 1) / SC_CHUNK;
+    __shared__ int sums[SC_TB];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
     __syncthreads();
-    for (int o = 1; o < nt; o <<= 1) {
-        int v = (tid >= o) ? sums[tid - o] : 0;
+    for (int b = 0; b < nch; b += SC_TB) {
+        const int q = b + threadIdx.x;
+        const int x = q < nch ? R.chunk_off[q] : 0;
+        sums[threadIdx.x] = x;
         __syncthreads();
-        sums[tid] += v;
+        for (int o = 1; o < SC_TB; o <<= 1) {
+            int y = threadIdx.x >= o ? sums[threadIdx.x - o] : 0;
+            __syncthreads();
+            sums[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (q < nch) R.chunk_off[q] = carry + sums[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 0) carry += sums[SC_TB - 1];
         __syncthreads();
     }
-    int run = sums[tid] - s;  // exclusive prefix of this chunk
-    for (int c = b; c < e; c++) {
-        int k = R.cell_cnt[c];
-        R.cell_start[c] = run;
-        R.cell_cnt[c] = 0;
-        run += k;
+}
+
+// R3: each old slot's cell position; the cell-sorted slot gets the
+// particle's original index as its sort key.
+__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, int ntiles)
+{
+    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[it.cur];
+        const Rep R = reps[t.rep];
+        if (!need_rebuild(R)) continue;
+        const int i = t.first + threadIdx.x;
+        if (i >= R.n) continue;
+        R.key[cstart(R, R.cell_of[i]) + R.rank[i]] = R.orig[i];
     }
-    if (tid == nt - 1) R.cell_start[nc] = sums[nt - 1];
 }
 
-// R3: scatter old slots into cell order.
-__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps)
+// R4: final slot = cell start + rank of the original index among the cell's
+// members (so the order within a cell is by original index: the slot order
+// is a pure function of the positions), and the live state (r/D, v, orig)
+// is scattered there.  F is dead at this point of the step (recomputed next)
+// and disp is zeroed by the build.
+__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, int ntiles)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
-    if (!R.dyn->rebuild) return;
-    const int i = t.first + threadIdx.x;
-    if (i >= R.n) return;
-    R.perm[R.cell_start[R.cell_of[i]] + R.rank[i]] = i;
+    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[it.cur];
+        const Rep R = reps[t.rep];
+        if (!need_rebuild(R)) continue;
+        const int x = t.first + threadIdx.x, n = R.n;
+        if (x >= n) continue;
+        const int c = R.cell_of[x];
+        const int s0 = cstart(R, c), e0 = cstart(R, c + 1);
+        const int oid = R.orig[x];
+        int rank = 0;
+        for (int a = s0; a < e0; a++) rank += R.key[a] < oid ? 1 : 0;
+        const int k = s0 + rank;
+        R.pos_alt[k] = R.pos[x];
+        for (int d = 0; d < R.dims; d++) R.v_alt[d * n + k] = R.v[d * n + x];
+        R.orig_alt[k] = oid;
+    }
 }
 
-// R4: order each cell's members by original index, so the new slot order
-// (and everything downstream) is a pure function of the positions.
-__global__ __launch_bounds__(SC_TB) void k_cellsort(const Tile *tiles, const Rep *reps)
+// R5: stencil sweep writing the symmetric short list.  For every slot it
+// counts the full-list partners (d2 <= R_v^2, add_pair!'s caller condition)
+// for the maxneighbours overflow check and appends each pair that also
+// satisfies the half-list predicate d2 < (r_cutoff*sigma_ij + skin)^2
+// (src/NeighborLists.jl:92).  It copies the permuted state back and zeroes
+// the displacements (src/NeighborLists.jl:35-42).
+//
+// Cell mode (stencil +-1): one wavefront per run of up to R.cx cells along x
+// stages the (cx+2) x 3^(dims-1) window cells' particles in LDS once, then
+// every own particle sweeps its 3^dims cells from LDS.  Windows larger than
+// SC_WMAX particles read the same candidates from global memory instead.
+template <int DIMS>
+__device__ inline void build_one(const Rep &R, int k, const double4 p, int &nfull, int &cnt, int j, const double4 q)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
-    if (!R.dyn->rebuild) return;
-    const int stride = R.ntiles * SC_TB;
-    for (int c = t.local * SC_TB + threadIdx.x; c < R.ncell; c += stride) {
-        const int b = R.cell_start[c], e = R.cell_start[c + 1];
-        for (int a = b + 1; a < e; a++) {
-            int x = R.perm[a];
-            int kx = R.orig[x];
-            int q = a - 1;
-            while (q >= b && R.orig[R.perm[q]] > kx) {
-                R.perm[q + 1] = R.perm[q];
-                q--;
-            }
-            R.perm[q + 1] = x;
+    double dx, dy, dz;
+    double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
+    if (d2 <= R.Rv2) {
+        nfull++;
+        double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
+        double rh = R.rcut * md + R.skin;
+        if (d2 < rh * rh) {
+            if (cnt < R.cap) R.nbr[(size_t)cnt * R.n + k] = j;
+            cnt++;
         }
     }
 }
 
-// R5: gather the live state into the new slot order (alternate buffers).
-// F is dead at this point of the step (recomputed next) and disp is zeroed,
-// so only r/D, v and the original index move.
-__global__ __launch_bounds__(SC_TB) void k_permute(const Tile *tiles, const Rep *reps)
+template <int DIMS>
+__device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, int nfull, int cnt)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
-    if (!R.dyn->rebuild) return;
-    const int k = t.first + threadIdx.x, n = R.n;
-    if (k >= n) return;
-    const int old = R.perm[k];
-    R.pos_alt[k] = R.pos[old];
-    for (int c = 0; c < R.dims; c++) R.v_alt[c * n + k] = R.v[c * n + old];
-    R.orig_alt[k] = R.orig[old];
+    const int n = R.n;
+    if (nfull > R.cap) atomicOr(&R.dyn->err, SC_EBIT_OVERFLOW);
+    R.nn[k] = cnt < R.cap ? cnt : R.cap;
+    R.pos[k] = p;
+    for (int c = 0; c < DIMS; c++) {
+        R.v[c * n + k] = R.v_alt[c * n + k];
+        R.disp[c * n + k] = 0.0;
+    }
+    R.orig[k] = R.orig_alt[k];
 }
 
-// R6: stencil sweep.  For every slot: count the full-list partners
-// (d2 <= R_v^2, add_pair!'s caller condition) for the maxneighbours
-// overflow check and append each pair that also satisfies the half-list
-// predicate d2 < (r_cutoff*sigma_ij + skin)^2 (src/NeighborLists.jl:92) to
-// the symmetric short list.  Copies the permuted state back and zeroes the
-// displacements (src/NeighborLists.jl:35-42).
+// LPP lanes cooperate on one particle: each takes every LPP-th candidate of
+// a row range, and the accepted entries are compacted with a ballot in
+// candidate order, so the list is identical to a serial sweep's.
+struct BuildLds {
+    double x[SC_WMAX], y[SC_WMAX], z[SC_WMAX], d[SC_WMAX];  // SoA: conflict-free 8-B reads
+    int j[SC_WMAX];
+    int coff[WAVE + 1];
+    int cglob[WAVE];
+};
+
+template <int DIMS, int LPP>
+__device__ void build_run(const Rep &R, int run, BuildLds &S)
+{
+    const int nc = R.nc, CX = R.cx, tid = threadIdx.x;
+    const int rpr = (nc + CX - 1) / CX;
+    const int rx = run % rpr, rest = run / rpr;
+    const int cy = rest % nc, cz = DIMS == 3 ? rest / nc : 0;
+    const int x0 = rx * CX, cxr = min(CX, nc - x0);
+    const int wx = cxr + 2;
+    const int ncw = wx * (DIMS == 3 ? 9 : 3);
+    __syncthreads();  // the previous run's readers are done with the LDS
+    if (tid < WAVE) {
+        int cnt = 0, gs = 0;
+        if (tid < ncw) {
+            const int w = tid / wx, xi = tid % wx;
+            const int oy = (w % 3) - 1, oz = DIMS == 3 ? (w / 3) - 1 : 0;
+            const int x = wrapc(x0 - 1 + xi, nc), y = wrapc(cy + oy, nc), z = DIMS == 3 ? wrapc(cz + oz, nc) : 0;
+            const int c = x + nc * (y + nc * z);
+            gs = cstart(R, c);
+            cnt = cstart(R, c + 1) - gs;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(incl, o, WAVE);
+            if (tid >= o) incl += y;
+        }
+        if (tid < ncw) {
+            S.coff[tid] = incl - cnt;
+            S.cglob[tid] = gs;
+        }
+        if (tid == WAVE - 1) S.coff[ncw] = incl;
+    }
+    __syncthreads();
+    const int total = S.coff[ncw];
+    const bool staged = total <= SC_WMAX;
+    if (staged) {
+        for (int a = tid; a < total; a += SC_TB) {
+            int lo = 0, hi = ncw - 1;  // last window cell q with S.coff[q] <= a
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (S.coff[mid] <= a)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int j = S.cglob[lo] + (a - S.coff[lo]);
+            const double4 q = R.pos_alt[j];
+            S.x[a] = q.x;
+            S.y[a] = q.y;
+            if (DIMS == 3) S.z[a] = q.z;
+            S.d[a] = q.w;
+            S.j[a] = j;
+        }
+    }
+    __syncthreads();
+    const int wc = DIMS == 3 ? 4 : 1;  // window row of the run itself
+    const int own0 = S.cglob[wc * wx + 1];
+    const int own1 = own0 + (S.coff[wc * wx + cxr + 1] - S.coff[wc * wx + 1]);
+    const int grp = tid / LPP, l = tid % LPP;
+    const int lane = tid % WAVE;
+    const int gshift = (lane / LPP) * LPP;
+    const unsigned long long gmask = ((1ull << LPP) - 1ull) << gshift;
+    const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
+    for (int base = own0; base < own1; base += SC_TB / LPP) {
+        const int k = base + grp;
+        if (k >= own1) continue;
+        const double4 p = R.pos_alt[k];
+        int cc[3];
+        cell_of_pos<DIMS>(R, p, cc);
+        const int xi = cc[0] - x0 + 1;
+        const int aself = S.coff[wc * wx + 1] + (k - own0);  // own window index
+        int nfull = 0, cnt = 0;
+        for (int w = 0; w < (DIMS == 3 ? 9 : 3); w++) {
+            const int q0 = w * wx + xi - 1;
+            const int a0 = S.coff[q0], a1 = S.coff[q0 + 3];
+            for (int a = a0 + l; a - l < a1; a += LPP) {
+                bool inF = false, inH = false;
+                int j = -1;
+                if (a < a1 && a != aself) {
+                    double qx, qy, qz = 0.0;
+                    if (staged) {
+                        qx = S.x[a];
+                        qy = S.y[a];
+                        if (DIMS == 3) qz = S.z[a];
+                    } else {
+                        const int cq = a < S.coff[q0 + 1] ? q0 : (a < S.coff[q0 + 2] ? q0 + 1 : q0 + 2);
+                        j = S.cglob[cq] + (a - S.coff[cq]);
+                        const double4 q = R.pos_alt[j];
+                        qx = q.x;
+                        qy = q.y;
+                        qz = q.z;
+                    }
+                    const double dx = min_image(p.x - qx, R.L, R.invL);
+                    const double dy = min_image(p.y - qy, R.L, R.invL);
+                    double d2 = dx * dx + dy * dy;
+                    if (DIMS == 3) {
+                        const double dz = min_image(p.z - qz, R.L, R.invL);
+                        d2 = d2 + dz * dz;
+                    }
+                    inF = d2 <= R.Rv2;
+                    if (inF) {
+                        double qd;
+                        if (staged) {
+                            qd = S.d[a];
+                            j = S.j[a];
+                        } else {
+                            qd = R.pos_alt[j].w;
+                        }
+                        const double md = (R.pot == 2) ? (p.w + qd) * 0.5 * (1.0 - R.zeta * fabs(p.w - qd)) : R.lj_md;
+                        const double rh = R.rcut * md + R.skin;
+                        inH = d2 < rh * rh;
+                    }
+                }
+                const unsigned long long mF = __ballot(inF) & gmask;
+                const unsigned long long mH = __ballot(inH) & gmask;
+                if (inH) {
+                    const int slot = cnt + __popcll(mH & below);
+                    if (slot < R.cap) R.nbr[(size_t)slot * R.n + k] = j;
+                }
+                nfull += __popcll(mF);
+                cnt += __popcll(mH);
+            }
+        }
+        if (l == 0) build_finish_slot<DIMS>(R, k, p, nfull, cnt);
+    }
+}
+
+__global__ __launch_bounds__(SC_TB) void k_build_cells(const Rep *reps, int nreps, int nruns_total)
+{
+    __shared__ BuildLds S;
+    for (XcdRange it = xcd_range(nruns_total); it.cur < it.end; it.cur += it.step) {
+        int r = 0;
+        while (r + 1 < nreps && reps[r + 1].run0 <= it.cur) r++;
+        const Rep R = reps[r];
+        if (!R.lds_build || !need_rebuild(R)) continue;
+        if (R.dims == 3)
+            build_run<3, 8>(R, it.cur - R.run0, S);
+        else
+            build_run<2, 4>(R, it.cur - R.run0, S);
+    }
+}
+
+// Cell-image displacement.  With a stencil of +-s cells (cell side w > R_v/s,
+// at least 2s+1 cells per side), a candidate in the neighbour cell reached by
+// a wrapped offset is displaced by dx_raw - S*L with S in {-1,0,1} the wrap
+// of that offset.  For every candidate that can pass d2 <= R_v^2, rint(dx_raw/L)
+// == S exactly (|dx_raw - S*L| <= R_v < L/2), so the result is bit-identical to
+// dx_raw - rint(dx_raw/L)*L (src/AuxiliaryFunctions.jl:313); candidates for
+// which the two images differ are farther than L - (2s+1)w/... > R_v under both
+// (|dx_raw - S*L| <= (s+1)w < L - R_v), so no predicate changes.
+template <int DIMS>
+__device__ inline void build_one_img(const Rep &R, int k, const double4 p, int &nfull, int &cnt, int j,
+                                     const double4 q, double sx, double sy, double sz)
+{
+    const double dx = (p.x - q.x) - sx;
+    const double dy = (p.y - q.y) - sy;
+    double d2 = dx * dx + dy * dy;
+    if (DIMS == 3) {
+        const double dz = (p.z - q.z) - sz;
+        d2 = d2 + dz * dz;
+    }
+    if (d2 <= R.Rv2) {
+        nfull++;
+        const double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
+        const double rh = R.rcut * md + R.skin;
+        if (d2 < rh * rh) {
+            if (cnt < R.cap) R.nbr[(size_t)cnt * R.n + k] = j;
+            cnt++;
+        }
+    }
+}
+
+// wrapped cell coordinate and the image shift S*L of the wrap
+__device__ inline int wrap_shift(int raw, int nc, double L, double &sL)
+{
+    if (raw < 0) {
+        sL = -L;
+        return raw + nc;
+    }
+    if (raw >= nc) {
+        sL = L;
+        return raw - nc;
+    }
+    sL = 0.0;
+    return raw;
+}
+
+// Per-slot build (stencil +-srange, or all pairs when srange == 0): one
+// thread per slot sweeping the stencil cells with global loads.
 template <int DIMS>
 __device__ void build_tile(const Rep &R, const Tile &t)
 {
     const int n = R.n, k = t.first + threadIdx.x;
-    if (k < n) {
-        const double4 p = R.pos_alt[k];
-        int cc[3];
-        cell_of_pos<DIMS>(R, p, cc);
-        const int s = R.srange;
-        int nfull = 0, cnt = 0;
-        const int zr = DIMS == 3 ? s : 0;
-        for (int oz = -zr; oz <= zr; oz++)
-            for (int oy = -s; oy <= s; oy++)
-                for (int ox = -s; ox <= s; ox++) {
-                    const int c = wrapc(cc[0] + ox, R.nc) +
-                                  R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
-                    const int b = R.cell_start[c], e = R.cell_start[c + 1];
-                    for (int j = b; j < e; j++) {
-                        if (j == k) continue;
-                        const double4 q = R.pos_alt[j];
-                        double dx, dy, dz;
-                        double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
-                        if (d2 <= R.Rv2) {
-                            nfull++;
-                            double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
-                            double rh = R.rcut * md + R.skin;
-                            if (d2 < rh * rh) {
-                                if (cnt < R.cap) R.nbr[(size_t)cnt * n + k] = j;
-                                cnt++;
-                            }
-                        }
-                    }
-                }
-        if (nfull > R.cap) atomicOr(&R.dyn->err, SC_EBIT_OVERFLOW);
-        R.nn[k] = cnt < R.cap ? cnt : R.cap;
-        R.pos[k] = p;
-        for (int c = 0; c < DIMS; c++) {
-            R.v[c * n + k] = R.v_alt[c * n + k];
-            R.disp[c * n + k] = 0.0;
-        }
-        R.orig[k] = R.orig_alt[k];
+    if (k >= n) return;
+    const double4 p = R.pos_alt[k];
+    int nfull = 0, cnt = 0;
+    if (R.srange == 0) {  // single cell: every other slot, reference minimum image
+        for (int j = 0; j < n; j++)
+            if (j != k) build_one<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j]);
+        build_finish_slot<DIMS>(R, k, p, nfull, cnt);
+        return;
     }
+    int cc[3];
+    cell_of_pos<DIMS>(R, p, cc);
+    const int s = R.srange, nc = R.nc;
+    const int zr = DIMS == 3 ? s : 0;
+    for (int oz = -zr; oz <= zr; oz++) {
+        double sz = 0.0;
+        const int cz = DIMS == 3 ? wrap_shift(cc[2] + oz, nc, R.L, sz) : 0;
+        for (int oy = -s; oy <= s; oy++) {
+            double sy;
+            const int cy = wrap_shift(cc[1] + oy, nc, R.L, sy);
+            for (int ox = -s; ox <= s; ox++) {
+                double sx;
+                const int cx = wrap_shift(cc[0] + ox, nc, R.L, sx);
+                const int c = cx + nc * (cy + nc * cz);
+                const int b = cstart(R, c), e = cstart(R, c + 1);
+                for (int j = b; j < e; j++) {
+                    if (j == k) continue;
+                    build_one_img<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j], sx, sy, sz);
+                }
+            }
+        }
+    }
+    build_finish_slot<DIMS>(R, k, p, nfull, cnt);
 }
 
-__global__ __launch_bounds__(SC_TB) void k_build(const Tile *tiles, const Rep *reps)
+__global__ __launch_bounds__(SC_TB) void k_build(const Tile *tiles, const Rep *reps, int ntiles)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
-    RepDyn *dyn = R.dyn;
-    if (!dyn->rebuild) return;
-    if (R.dims == 3)
-        build_tile<3>(R, t);
-    else
-        build_tile<2>(R, t);
+    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[it.cur];
+        const Rep R = reps[t.rep];
+        if (R.lds_build || !need_rebuild(R)) continue;
+        if (R.dims == 3)
+            build_tile<3>(R, t);
+        else
+            build_tile<2>(R, t);
+    }
 }
 
 // Full rows at the current positions over the rebuild-time cells, for the
@@ -773,7 +1144,7 @@ __device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *coun
             for (int ox = -s; ox <= s; ox++) {
                 const int c = wrapc(cc[0] + ox, R.nc) +
                               R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
-                for (int j = R.cell_start[c]; j < R.cell_start[c + 1]; j++) {
+                for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
                     if (j == k) continue;
                     double dx, dy, dz;
                     double d2 = sqdist<DIMS>(R, p, R.pos[j], dx, dy, dz);
@@ -788,8 +1159,8 @@ __device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *coun
 
 __global__ __launch_bounds__(SC_TB) void k_full_rows(const Tile *tiles, const Rep *reps, int *rows, int *counts, int cap)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     if (R.dims == 3)
         full_rows_tile<3>(R, t, rows, counts, cap);
     else
@@ -821,7 +1192,7 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
                 for (int ox = -s; ox <= s; ox++) {
                     const int c = wrapc(cc[0] + ox, R.nc) +
                                   R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
-                    for (int j = R.cell_start[c]; j < R.cell_start[c + 1]; j++) {
+                    for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
                         if (j == k) continue;
                         const double4 q = R.pos[j];
                         double dx, dy, dz;
@@ -838,15 +1209,15 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
     if (threadIdx.x == 0) {
         double sum = 0.0, comp = 0.0;
         for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
-        R.part[t.local * 8 + PS_ESUM] = sum;
-        R.part[t.local * 8 + PS_ECOMP] = comp;
+        R.part[PS_ESUM * R.ntiles + t.local] = sum;
+        R.part[PS_ECOMP * R.ntiles + t.local] = comp;
     }
 }
 
 __global__ __launch_bounds__(SC_TB) void k_energy(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     const int key = (R.dims == 3 ? 2 : 0) + (R.pot == 2 ? 1 : 0);
     switch (key) {
     case 0: energy_tile<2, 1>(R, t); break;
@@ -885,15 +1256,15 @@ __device__ void energy_nn_tile(const Rep &R, const Tile &t)
     if (threadIdx.x == 0) {
         double sum = 0.0, comp = 0.0;
         for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
-        R.part[t.local * 8 + PS_ESUM] = sum;
-        R.part[t.local * 8 + PS_ECOMP] = comp;
+        R.part[PS_ESUM * R.ntiles + t.local] = sum;
+        R.part[PS_ECOMP * R.ntiles + t.local] = comp;
     }
 }
 
 __global__ __launch_bounds__(SC_TB) void k_energy_nn(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     const int key = (R.dims == 3 ? 2 : 0) + (R.pot == 2 ? 1 : 0);
     switch (key) {
     case 0: energy_nn_tile<2, 1>(R, t); break;
@@ -907,8 +1278,8 @@ __global__ __launch_bounds__(SC_TB) void k_energy_nn(const Tile *tiles, const Re
 // also stages rescale_velocities! (src/MD.jl:391-402) as a pending scale.
 __global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *reps)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     RepDyn *dyn = R.dyn;
     const int n = R.n, i = t.first + threadIdx.x;
     double e[1] = {0.0};
@@ -929,8 +1300,8 @@ __global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *re
 // ====================================================================
 __global__ __launch_bounds__(SC_TB) void k_to_host(const Tile *tiles, const Rep *reps, int field)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     const int n = R.n, k = t.first + threadIdx.x, d = R.dims;
     if (k >= n) return;
     const int o = R.orig[k];
@@ -962,8 +1333,8 @@ __global__ __launch_bounds__(SC_TB) void k_to_host(const Tile *tiles, const Rep 
 
 __global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Rep *reps, int field)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     const int n = R.n, k = t.first + threadIdx.x, d = R.dims;
     if (k >= n) return;
     const int o = R.orig[k];
@@ -996,8 +1367,8 @@ __global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Re
 // ====================================================================
 __global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Rep *reps, double dt)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     RepDyn *dyn = R.dyn;
     const int n = R.n, i = t.first + threadIdx.x;
     double st = 0.0, fm = 0.0;
@@ -1047,13 +1418,13 @@ __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
         s2 = s;
         if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
     }
-    put_partial(R, t, PS_MAXD2, block_max(s2));
+    publish_maxdisp(R, block_max(s2));
 }
 
 __global__ __launch_bounds__(SC_TB) void k_min_move(const Tile *tiles, const Rep *reps, double dt)
 {
-    const Tile t = tiles[blockIdx.x];
-    const Rep &R = reps[t.rep];
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
     if (R.dims == 3)
         min_move_tile<3>(R, t, dt);
     else
@@ -1082,21 +1453,14 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
 {
     const int r = blockIdx.x;
     if (r >= nreps) return;
-    const Rep &R = reps[r];
+    const Rep R = reps[r];
     if (!((sysmask >> R.sys) & 1)) return;
     RepDyn *dyn = R.dyn;
     const int tid = threadIdx.x;
     const double *P = R.part;
     switch (op) {
-    case FIN_DRIFT: {
-        double m = 0.0;
-        for (int q = tid; q < R.ntiles; q += SC_TB) m = fmax(m, P[q * 8 + PS_MAXD2]);
-        m = block_max(m);
-        if (tid == 0) {
-            if (m > R.thr) dyn->rebuild = 1;
-            dyn->pending = 0;
-        }
-    } break;
+    case FIN_DRIFT:
+        break;
     case FIN_FORCE0:
         if (tid == 0) dyn->step += 1;
         break;
@@ -1104,7 +1468,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     case FIN_FORCE2: {
         double a[3] = {0.0, 0.0, 0.0};
         for (int q = tid; q < R.ntiles; q += SC_TB)
-            for (int c = 0; c < 3; c++) a[c] += P[q * 8 + PS_VEC + c];
+            for (int c = 0; c < 3; c++) a[c] += P[(PS_VEC + c) * R.ntiles + q];
         block_sum<3>(a);
         if (tid == 0) {
             for (int c = 0; c < R.dims; c++) {
@@ -1114,7 +1478,16 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
                 else
                     dyn->Fmean[c] = mean;
             }
-            if (op == FIN_FORCE1) dyn->pending = 1 | (R.rescale ? 2 : 0);
+            if (op == FIN_FORCE1) {
+                // rebuild bookkeeping of this step (the rebuild kernels ran
+                // between the drift and this force evaluation)
+                if (dyn->rebuild) {
+                    dyn->rebuilds += 1;
+                    dyn->have_lists = 1;
+                }
+                dyn->rebuild = 0;
+                dyn->pending = 1 | (R.rescale ? 2 : 0);
+            }
             dyn->step += 1;
         }
     } break;
@@ -1122,7 +1495,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     case FIN_EKIN:
     case FIN_EKIN_RESCALE: {
         double a[1] = {0.0};
-        for (int q = tid; q < R.ntiles; q += SC_TB) a[0] += P[q * 8 + PS_SCAL];
+        for (int q = tid; q < R.ntiles; q += SC_TB) a[0] += P[PS_SCAL * R.ntiles + q];
         block_sum<1>(a);
         if (tid == 0) {
             double ek = a[0] * (R.m / 2.0);
@@ -1142,18 +1515,21 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
         if (tid == 0) dyn->pending = 0;
         break;
     case FIN_REBUILD_DONE:
-        if (tid == 0 && dyn->rebuild) {
+        if (tid == 0) {
+            if (dyn->rebuild) {
+                dyn->rebuilds += 1;
+                dyn->have_lists = 1;
+            }
             dyn->rebuild = 0;
-            dyn->rebuilds += 1;
-            dyn->have_lists = 1;
+            dyn->pending = 0;
         }
         break;
     case FIN_ENERGY:
         if (tid == 0) {
             double sum = 0.0, comp = 0.0;
             for (int q = 0; q < R.ntiles; q++) {
-                neumaier_add(sum, comp, P[q * 8 + PS_ESUM]);
-                comp += P[q * 8 + PS_ECOMP];
+                neumaier_add(sum, comp, P[PS_ESUM * R.ntiles + q]);
+                comp += P[PS_ECOMP * R.ntiles + q];
             }
             dyn->result[0] = (sum + comp) / 2.0;
         }
@@ -1161,8 +1537,8 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     case FIN_MINSCALE: {
         double st = 0.0, fm = 0.0;
         for (int q = tid; q < R.ntiles; q += SC_TB) {
-            st = fmax(st, P[q * 8 + PS_AUXMAX]);
-            fm = fmax(fm, P[q * 8 + PS_VEC]);
+            st = fmax(st, P[PS_AUXMAX * R.ntiles + q]);
+            fm = fmax(fm, P[PS_VEC * R.ntiles + q]);
         }
         st = block_max(st);
         fm = block_max(fm);
@@ -1195,7 +1571,6 @@ int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps,
 {
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    fin(reps, nreps, FIN_DRIFT, SYS_VV, 0.0, s);
     return check(hipGetLastError());
 }
 
@@ -1228,15 +1603,17 @@ int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipSt
 
 int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s)
 {
-    fin(reps, nreps, FIN_DRIFT, sysmask, 0.0, s);
-    return check(hipGetLastError());
+    (void)reps;
+    (void)nreps;
+    (void)sysmask;
+    (void)s;
+    return 0;  // the drift publishes its maximum with atomics; nothing to finish
 }
 
 int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_drift_brownian, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    fin(reps, nreps, FIN_DRIFT, SYS_BD, 0.0, s);
     return check(hipGetLastError());
 }
 
@@ -1254,16 +1631,22 @@ int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s)
     return check(hipGetLastError());
 }
 
-int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
+                   bool bookkeep)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_bin, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    hipLaunchKernelGGL(k_scan, dim3(nreps), dim3(1024), 0, s, reps, nreps);
-    hipLaunchKernelGGL(k_scatter, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    hipLaunchKernelGGL(k_cellsort, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    hipLaunchKernelGGL(k_permute, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    hipLaunchKernelGGL(k_build, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
-    fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
+    const int gt = ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP;
+    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
+    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, nreps);
+    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, nreps);
+    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
+    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
+    if (g.nruns > 0) {
+        const int gr = g.nruns < SC_RUN_GRID_CAP ? g.nruns : SC_RUN_GRID_CAP;
+        hipLaunchKernelGGL(k_build_cells, dim3(gr), dim3(SC_TB), 0, s, reps, nreps, g.nruns);
+    }
+    if (g.any_tile_build) hipLaunchKernelGGL(k_build, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
+    if (bookkeep) fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }
 
@@ -1333,7 +1716,6 @@ int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, int nreps, d
 {
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_min_move, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, dt);
-    fin(reps, nreps, FIN_DRIFT, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }
 

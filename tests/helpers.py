@@ -96,7 +96,7 @@ def relax_oracle(op, r, D, iters=300, step=0.02):
     return np.ascontiguousarray(r)
 
 
-def sc_params_from(op, rc, precision=nat.PREC_FP64, cell_divisor=1, seed=1234):
+def sc_params_from(op, rc, precision=nat.PREC_FP64, cell_divisor=1, seed=1234, build_mode=0):
     p = nat.ScParams()
     for f, _ in nat.ScParams._fields_:
         if hasattr(op, f):
@@ -104,6 +104,7 @@ def sc_params_from(op, rc, precision=nat.PREC_FP64, cell_divisor=1, seed=1234):
     p.force_cutoff = rc
     p.precision = precision
     p.cell_divisor = cell_divisor
+    p.build_mode = build_mode
     p.seed = seed
     return p
 

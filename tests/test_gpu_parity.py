@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 CASES = [("lj", 3, 1000), ("berthier", 3, 512), ("berthier", 2, 900), ("lj", 2, 400), ("berthier", 3, 125)]
 
 
-def make_ctx(op, rc, r, v, D, precision=nat.PREC_FP64, cell_divisor=1, seed=1234):
-    ctx = nat.Context(H.sc_params_from(op, rc, precision, cell_divisor, seed), D)
+def make_ctx(op, rc, r, v, D, precision=nat.PREC_FP64, cell_divisor=1, seed=1234, build_mode=0):
+    ctx = nat.Context(H.sc_params_from(op, rc, precision, cell_divisor, seed, build_mode), D)
     ctx.set_state(r=r, v=v)
     return ctx
 
@@ -30,11 +30,11 @@ def case_id(c):
 
 
 @pytest.mark.parametrize("case", CASES, ids=case_id)
-@pytest.mark.parametrize("cell_divisor", [1, 2])
-def test_pair_sets_bit_exact(case, cell_divisor):
+@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (3, 0), (1, 1)])
+def test_pair_sets_bit_exact(case, cell_divisor, build_mode):
     kind, dims, N = case
     op, r, D, v, rc = H.make_system(kind, N, dims, seed=100 + N)
-    ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor)
+    ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=build_mode)
     ctx.update_neighbor_lists()
     nf, full, nh, half = ctx.get_neighbor_lists()
     lst = oracle.build_lists(op, r, D, cells=False)
@@ -77,10 +77,10 @@ def test_energies(case):
     assert abs(ctx.kinetic_energy() - ek) <= 1e-13 * ek
 
 
-def _trajectory(kind, dims, N, system, steps, rescale=False, dt=0.005, cell_divisor=1):
+def _trajectory(kind, dims, N, system, steps, rescale=False, dt=0.005, cell_divisor=1, build_mode=0):
     op, r, D, v, rc = H.make_system(kind, N, dims, seed=400 + N + system, system=system, dt=dt,
                                     kBT=0.2, rescale=rescale)
-    ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor)
+    ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=build_mode)
     ctx.update_neighbor_lists()
     st = oracle.State(op, r, v, D)
     st.update_neighbor_lists()
@@ -116,9 +116,10 @@ def test_trajectory_rescale_temperature():
     assert np.max(np.abs(g["v"] - st.v)) <= 1e-10
 
 
-def test_trajectory_many_rebuilds_cell_divisor_2():
+@pytest.mark.parametrize("cell_divisor,build_mode", [(2, 0), (1, 1)])
+def test_trajectory_many_rebuilds(cell_divisor, build_mode):
     op, st, g, dr, ctx = _trajectory("berthier", 3, 1000, oracle.SYS_LANGEVIN, 400, dt=0.01,
-                                      cell_divisor=2)
+                                      cell_divisor=cell_divisor, build_mode=build_mode)
     assert st.rebuilds[0] >= 5
     assert ctx.rebuild_count() == st.rebuilds[0]
     assert dr <= 1e-10

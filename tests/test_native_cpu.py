@@ -30,8 +30,8 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_params_struct_matches_header_size():
-    # sc_params: 6 int32, 2 int64, 4 + 3 + 4 + 4 doubles, one uint64
-    assert ctypes.sizeof(nat.ScParams) == 6 * 4 + 2 * 8 + 15 * 8 + 8
+    # sc_params: 8 int32, 2 int64, 4 + 3 + 4 + 4 doubles, one uint64
+    assert ctypes.sizeof(nat.ScParams) == 8 * 4 + 2 * 8 + 15 * 8 + 8
 
 
 def test_create_rejects_bad_arguments_without_touching_the_device():
