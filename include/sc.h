@@ -155,6 +155,45 @@ int sc_kernel_times(sc_ctx *ctx, double *force_ms, int64_t *force_launches, doub
  * counted from both sides) and the longest row. */
 int sc_neighbor_stats(sc_ctx *ctx, int64_t *total_entries, int64_t *max_row);
 
+/* ------------------------------------------------------------------------
+ * Replica batches (no reference counterpart: the reference runs one system
+ * per Julia process).  Independent systems of any N, dims, potential,
+ * dynamics and precision are admitted into a batch and advanced together,
+ * each kernel of a step launched once for all of them; the step may be
+ * captured into a hipGraph.  Replicas draw on-device Philox noise keyed by
+ * their own sc_params.seed, so a replica's trajectory is bitwise identical
+ * alone and in any mix.  Each replica follows the reference step of its own
+ * dynamics (src/MD.jl:429-554). */
+typedef struct sc_batch sc_batch;
+
+int sc_batch_create(sc_batch **out, int32_t max_replicas);
+void sc_batch_destroy(sc_batch *b);
+const char *sc_batch_last_error(const sc_batch *b);
+/* Device bytes a replica with these parameters and diameters would occupy
+ * (for admission by HBM budget); -1 on bad arguments. */
+int64_t sc_replica_device_bytes(const sc_params *p, const double *D);
+/* Admit a replica: allocate its state, upload r (and v, may be NULL), build
+ * its lists.  *id identifies it until sc_batch_remove. */
+int sc_batch_add(sc_batch *b, const sc_params *p, const double *D, const double *r, const double *v,
+                 int32_t *id);
+int sc_batch_remove(sc_batch *b, int32_t id);
+/* Capture the step into a hipGraph (re-captured when the membership
+ * changes) and replay it, instead of launching kernel by kernel. */
+int sc_batch_set_graph(sc_batch *b, int32_t on);
+/* Advance every admitted replica by nsteps steps. */
+int sc_batch_step(sc_batch *b, int64_t nsteps);
+/* Capped steepest descent (sc_minimize) of every admitted replica. */
+int sc_batch_minimize(sc_batch *b, int64_t iters, double dt, double max_step);
+int sc_batch_get_state(sc_batch *b, int32_t id, double *r, double *v, double *F);
+/* Overwrite r, v and/or F of a replica (NULL = keep); lists are not rebuilt. */
+int sc_batch_set_state(sc_batch *b, int32_t id, const double *r, const double *v, const double *F);
+/* Steps taken in the batch, rebuild count, and (if epot != NULL) the
+ * potential energy of calculate_full_energy. */
+int sc_batch_replica_stats(sc_batch *b, int32_t id, int64_t *steps, int64_t *rebuilds, double *epot);
+/* Admitted replicas, their total particles and device bytes, and the device
+ * time of the last sc_batch_step in milliseconds. */
+int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *device_bytes, double *last_ms);
+
 #ifdef __cplusplus
 }
 #endif

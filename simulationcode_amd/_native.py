@@ -44,6 +44,9 @@ EXPORTS = (
     "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
     "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_philox4x32_10",
     "sc_last_elapsed_ms", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
+    "sc_batch_create", "sc_batch_destroy", "sc_batch_last_error", "sc_replica_device_bytes",
+    "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
+    "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
 )
 
 
@@ -127,6 +130,21 @@ def load(path=LIB_PATH):
         "sc_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "sc_kernel_times": (ctypes.c_int, [vp, pd, ctypes.POINTER(i64), pd, ctypes.POINTER(i64)]),
         "sc_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "sc_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int32]),
+        "sc_batch_destroy": (None, [vp]),
+        "sc_batch_last_error": (ctypes.c_char_p, [vp]),
+        "sc_replica_device_bytes": (i64, [ctypes.POINTER(ScParams), vp]),
+        "sc_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(ScParams), vp, vp, vp, ctypes.POINTER(ctypes.c_int32)]),
+        "sc_batch_remove": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "sc_batch_set_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "sc_batch_step": (ctypes.c_int, [vp, i64]),
+        "sc_batch_minimize": (ctypes.c_int, [vp, i64, dbl, dbl]),
+        "sc_batch_get_state": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
+        "sc_batch_set_state": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
+        "sc_batch_replica_stats": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(i64),
+                                                  pd]),
+        "sc_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64),
+                                         ctypes.POINTER(i64), pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -289,3 +307,109 @@ class Context:
         t, m = ctypes.c_int64(), ctypes.c_int64()
         self._check(self._lib.sc_neighbor_stats(self._h, ctypes.byref(t), ctypes.byref(m)))
         return t.value, m.value
+
+
+def replica_device_bytes(params: ScParams, D):
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    return int(load().sc_replica_device_bytes(ctypes.byref(params), D.ctypes.data))
+
+
+class Batch:
+    """Replicas advanced by shared launches (sc_batch_*)."""
+
+    def __init__(self, max_replicas=256, graph=False):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        rc = self._lib.sc_batch_create(ctypes.byref(h), int(max_replicas))
+        self._h = h
+        self._shape = {}
+        if rc != SC_OK:
+            msg = self._lib.sc_batch_last_error(h).decode() if h.value else "invalid arguments"
+            self.close()
+            raise ScError(rc, msg)
+        if graph:
+            self.set_graph(True)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.sc_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != SC_OK:
+            msg = self._lib.sc_batch_last_error(self._h).decode()
+            if rc == SC_ERR_OVERFLOW:
+                raise NeighborOverflowError(rc, msg)
+            raise ScError(rc, msg)
+
+    def add(self, params: ScParams, D, r, v=None):
+        D = np.ascontiguousarray(D, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        if D.shape != (params.N,) or r.size != params.N * params.dims:
+            raise ValueError("D / r shapes do not match params")
+        if v is not None:
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            if v.size != r.size:
+                raise ValueError("v shape does not match r")
+        rid = ctypes.c_int32()
+        self._check(self._lib.sc_batch_add(self._h, ctypes.byref(params), D.ctypes.data, r.ctypes.data,
+                                           None if v is None else v.ctypes.data, ctypes.byref(rid)))
+        self._shape[rid.value] = (int(params.N), int(params.dims))
+        return rid.value
+
+    def remove(self, rid):
+        self._check(self._lib.sc_batch_remove(self._h, int(rid)))
+        self._shape.pop(rid, None)
+
+    def set_graph(self, on):
+        self._check(self._lib.sc_batch_set_graph(self._h, 1 if on else 0))
+
+    def step(self, nsteps):
+        self._check(self._lib.sc_batch_step(self._h, int(nsteps)))
+
+    def minimize(self, iters, dt, max_step):
+        self._check(self._lib.sc_batch_minimize(self._h, int(iters), float(dt), float(max_step)))
+
+    def get_state(self, rid):
+        n, d = self._shape[rid]
+        r, v, F = np.zeros((n, d)), np.zeros((n, d)), np.zeros((n, d))
+        self._check(self._lib.sc_batch_get_state(self._h, int(rid), r.ctypes.data, v.ctypes.data, F.ctypes.data))
+        return {"r": r, "v": v, "F": F}
+
+    def set_state(self, rid, r=None, v=None, F=None):
+        n, d = self._shape[rid]
+        arrs = []
+        for a in (r, v, F):
+            if a is None:
+                arrs.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.size != n * d:
+                raise ValueError("array shape does not match the replica")
+            arrs.append(a)
+        self._check(self._lib.sc_batch_set_state(self._h, int(rid), *[None if a is None else a.ctypes.data
+                                                                     for a in arrs]))
+
+    def stats(self, rid, energy=False):
+        st, rb = ctypes.c_int64(), ctypes.c_int64()
+        e = ctypes.c_double()
+        self._check(self._lib.sc_batch_replica_stats(self._h, int(rid), ctypes.byref(st), ctypes.byref(rb),
+                                                     ctypes.byref(e) if energy else None))
+        out = {"steps": st.value, "rebuilds": rb.value}
+        if energy:
+            out["epot"] = e.value
+        return out
+
+    def info(self):
+        a = ctypes.c_int32()
+        npart, by = ctypes.c_int64(), ctypes.c_int64()
+        ms = ctypes.c_double()
+        self._check(self._lib.sc_batch_info(self._h, ctypes.byref(a), ctypes.byref(npart), ctypes.byref(by),
+                                            ctypes.byref(ms)))
+        return {"active": a.value, "particles": npart.value, "device_bytes": by.value, "last_ms": ms.value}

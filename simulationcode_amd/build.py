@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 OUT = os.path.join(OUT_DIR, "libsc.so")
-SOURCES = ["sc_kernels.hip", "sc_api.hip", "sc_batch.hip"]
+SOURCES = ["sc_kernels.hip", "sc_replica.hip", "sc_api.hip", "sc_batch.hip"]
 ARCH = os.environ.get("SC_OFFLOAD_ARCH", "gfx950")
 
 

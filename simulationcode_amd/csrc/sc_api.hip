@@ -165,142 +165,29 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
 {
     if (!out || !p || !D) return SC_ERR_ARG;
     *out = nullptr;
-    if ((p->dims != 2 && p->dims != 3) || (p->potential != SC_POT_LJ && p->potential != SC_POT_BERTHIER) ||
-        p->system < 0 || p->system > 2 || (p->precision != SC_PREC_FP64 && p->precision != SC_PREC_MIXED) ||
-        p->N < 2 || p->N > (int64_t)1 << 30 || p->maxneighbours < 1 || p->maxneighbours > 100000 ||
-        !(p->box_size > 0.0) || !(p->skin_distance >= 0.0) || !(p->force_cutoff > 0.0))
-        return SC_ERR_ARG;
+    if (!sc::params_valid(p)) return SC_ERR_ARG;
     sc_ctx *c = new sc_ctx();
     c->p = *p;
-    Rep &h = c->h;
-    const int n = (int)p->N;
-    double maxD = D[0];
-    for (int i = 1; i < n; i++) maxD = std::max(maxD, D[i]);
-    // initialize_neighbor_struct, src/DataStructs.jl:351-352
-    const double Rv = p->skin_distance + p->force_cutoff * maxD;
-    if (!(p->box_size > 2.0 * Rv)) {
-        int rc = fail(c, SC_ERR_ARG, "box_size must exceed twice the Verlet radius");
-        *out = c;
-        return rc;
-    }
-    h.dims = p->dims;
-    h.pot = p->potential;
-    h.sys = p->system;
-    h.prec = p->precision;
-    h.n = n;
-    h.cap = (int)p->maxneighbours;
-    h.rescale = (p->system == SC_SYS_NEWTONIAN && p->rescale_temperature) ? 1 : 0;
-    h.L = p->box_size;
-    h.invL = 1.0 / p->box_size;
-    h.fc2 = p->force_cutoff2;
-    h.rcut = std::sqrt(p->force_cutoff2);  // src/NeighborLists.jl:44
-    h.skin = p->skin_distance;
-    h.Rv2 = Rv * Rv;
-    h.thr = (p->skin_distance / 2.0) * (p->skin_distance / 2.0);
-    std::memcpy(&h.thr_bits, &h.thr, sizeof(double));
-    h.eps = p->lj_eps;
-    h.lj_s2 = p->lj_sigma * p->lj_sigma;
-    h.lj_md = p->lj_sigma * p->lj_sigma;  // find_mean_D for LJ returns sigma^2
-    h.Uc = p->lj_Uc;
-    h.c0 = p->b_c0;
-    h.c2 = p->b_c2;
-    h.c4 = p->b_c4;
-    h.zeta = p->b_zeta;
-    h.kBT = p->kBT;
-    h.gamma = p->gamma;
-    h.dt = p->dt;
-    h.m = p->m;
-    h.seed = p->seed;
-    // cells of side >= R_v / k with a (2k+1)^dims stencil; a hair of slack so
-    // floating-point cell assignment can never drop a pair at exactly R_v
-    const int k = p->cell_divisor > 0 ? p->cell_divisor : 2;
-    long long nc = (long long)std::floor(k * p->box_size / (Rv * (1.0 + 1e-10)));
-    int srange = k;
-    if (nc < 2 * k + 1) {
-        nc = 1;
-        srange = 0;
-    }
-    long long ncell = nc * nc * (p->dims == 3 ? nc : 1);
-    if (ncell > (1LL << 27)) {
-        int rc = fail(c, SC_ERR_ARG, "cell grid too large");
-        *out = c;
-        return rc;
-    }
-    h.nc = (int)nc;
-    h.ncell = (int)ncell;
-    h.srange = srange;
-    h.cinv = (double)nc / p->box_size;
-    c->ntiles = (n + SC_TB - 1) / SC_TB;
-    h.ntiles = c->ntiles;
-    h.tile0 = 0;
-    // LDS build runs: ~64 own particles per block, window <= 64 cells
-    h.lds_build = (srange == 1 && p->build_mode == 1) ? 1 : 0;
-    if (h.lds_build) {
-        const double per_cell = (double)n / (double)ncell;
-        const int cxmax = (p->dims == 3 ? 5 : 19);
-        int cx = (int)std::lround(64.0 / std::max(per_cell, 1e-9));
-        cx = std::max(1, std::min(cx, std::min(cxmax, (int)nc - 2)));
-        h.cx = cx;
-        const long long rpr = (nc + cx - 1) / cx;
-        h.nruns = (int)(rpr * nc * (p->dims == 3 ? nc : 1));
-    } else {
-        h.cx = 0;
-        h.nruns = 0;
-    }
-    h.run0 = 0;
-    c->geom.max_chunks = (int)((ncell + SC_CHUNK - 1) / SC_CHUNK);
-    c->geom.nruns = h.nruns;
-    c->geom.any_tile_build = h.lds_build ? 0 : 1;
     *out = c;
-
     if (hipStreamCreate(&c->s) != hipSuccess) return fail(c, SC_ERR_HIP, "hipStreamCreate failed");
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
-    const int d = p->dims;
-    int rc = SC_OK;
-    rc |= dalloc(c, &h.pos, n);
-    rc |= dalloc(c, &h.pos_alt, n);
-    rc |= dalloc(c, &h.v, (size_t)n * d);
-    rc |= dalloc(c, &h.v_alt, (size_t)n * d);
-    rc |= dalloc(c, &h.F, (size_t)n * d);
-    rc |= dalloc(c, &h.disp, (size_t)n * d);
-    rc |= dalloc(c, &h.orig, n);
-    rc |= dalloc(c, &h.orig_alt, n);
-    rc |= dalloc(c, &h.cell_of, n);
-    rc |= dalloc(c, &h.rank, n);
-    rc |= dalloc(c, &h.key, n);
-    rc |= dalloc(c, &h.chunk_off, (size_t)c->geom.max_chunks);
-    rc |= dalloc(c, &h.cell_cnt, (size_t)ncell);
-    rc |= dalloc(c, &h.cell_start, (size_t)ncell + 1);
-    rc |= dalloc(c, &h.nn, n);
-    rc |= dalloc(c, &h.nbr, (size_t)n * h.cap);
-    rc |= dalloc(c, &h.part, (size_t)c->ntiles * 8);
-    rc |= dalloc(c, &h.stage, (size_t)n * 3);
-    rc |= dalloc(c, &h.dyn, 1);
+    Rep &h = c->h;
+    int rc = sc::replica_init(h, p, D, c->allocs, c->err);
+    if (rc != SC_OK) return rc;
+    c->ntiles = h.ntiles;
+    h.tile0 = 0;
+    h.run0 = 0;
+    c->geom.max_chunks = (h.ncell + SC_CHUNK - 1) / SC_CHUNK;
+    c->geom.nruns = h.nruns;
+    c->geom.any_tile_build = h.lds_build ? 0 : 1;
+    rc = SC_OK;
     rc |= dalloc(c, &c->d_rep, 1);
     rc |= dalloc(c, &c->d_tiles, c->ntiles);
     if (rc != SC_OK) return SC_ERR_OOM;
-
     std::vector<Tile> tiles(c->ntiles);
     for (int t = 0; t < c->ntiles; t++) tiles[t] = Tile{0, t * SC_TB, t, 0};
-    std::vector<int> ident(n);
-    for (int i = 0; i < n; i++) ident[i] = i;
-    std::vector<double4> pos(n);
-    for (int i = 0; i < n; i++) pos[i] = make_double4(0.0, 0.0, 0.0, D[i]);
-    RepDyn dyn;
-    std::memset(&dyn, 0, sizeof(dyn));
-    dyn.vscale = 1.0;
     HIPCHK(c, hipMemcpy(c->d_tiles, tiles.data(), sizeof(Tile) * c->ntiles, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(h.orig, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(h.pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemset(h.v, 0, sizeof(double) * n * d));
-    HIPCHK(c, hipMemset(h.F, 0, sizeof(double) * n * d));
-    HIPCHK(c, hipMemset(h.disp, 0, sizeof(double) * n * d));
-    HIPCHK(c, hipMemset(h.cell_cnt, 0, sizeof(int) * ncell));
-    HIPCHK(c, hipMemset(h.nn, 0, sizeof(int) * n));
-    HIPCHK(c, hipMemset(h.part, 0, sizeof(double) * c->ntiles * 8));
-    HIPCHK(c, hipMemset(h.chunk_off, 0, sizeof(int) * c->geom.max_chunks));
-    HIPCHK(c, hipMemcpy(h.dyn, &dyn, sizeof(RepDyn), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_rep, &h, sizeof(Rep), hipMemcpyHostToDevice));
     return SC_OK;
 }

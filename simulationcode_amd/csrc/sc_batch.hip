@@ -1,0 +1,394 @@
+// sc_batch.hip -- many independent replicas advanced by shared launches.
+//
+// Every replica keeps its own device state (sc_replica.hip); the batch owns
+// one compact descriptor array and one tile table over all active replicas,
+// so each kernel of a step is launched once for the whole batch whatever the
+// replicas' N, dimension, potential, dynamics or precision.  A step is
+//   [Brownian: forces (+noise), mean force, drift]  [VV: first kick + drift]
+//   [rebuild where a replica's skin trigger fired]  [VV: forces, second kick,
+//   COM]  [rescale where requested]
+// and may be captured once into a hipGraph and replayed.  Replicas use the
+// on-device Philox noise keyed by their own seed, so a replica's trajectory is
+// bitwise the same alone and in any mix.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sc.h"
+#include "sc_internal.h"
+
+namespace {
+
+struct Slot {
+    bool used = false;
+    Rep h{};
+    sc_params p{};
+    std::vector<void *> allocs;
+    long long steps = 0;
+    long long bytes = 0;
+};
+
+}  // namespace
+
+struct sc_batch {
+    int max_reps = 0;
+    std::vector<Slot> slots;
+    hipStream_t s = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Rep *d_reps = nullptr;
+    Tile *d_tiles = nullptr;
+    int tiles_cap = 0, ntiles = 0, nact = 0;
+    std::vector<int> act;  // slot of each compact replica index
+    sc::RebuildGeom geom{};
+    bool dirty = true, has_vv = false, has_bd = false, has_rescale = false;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+    double last_ms = 0.0;
+};
+
+namespace {
+
+int bfail(sc_batch *b, int code, const std::string &m)
+{
+    if (b) b->err = m;
+    return code;
+}
+
+#define BCHK(b, expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return bfail(b, SC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define BK(b, expr)                                                                              \
+    do {                                                                                         \
+        if ((expr) != 0) return bfail(b, SC_ERR_HIP, std::string("kernel launch failed: ") +   \
+                                                         hipGetErrorString(hipGetLastError())); \
+    } while (0)
+
+void drop_graph(sc_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// Rebuild the compact descriptor array and the tile table after admissions
+// or retirements.
+int sync_tables(sc_batch *b)
+{
+    if (!b->dirty) return SC_OK;
+    BCHK(b, hipStreamSynchronize(b->s));
+    drop_graph(b);
+    b->act.clear();
+    for (int i = 0; i < (int)b->slots.size(); i++)
+        if (b->slots[i].used) b->act.push_back(i);
+    b->nact = (int)b->act.size();
+    std::vector<Rep> reps;
+    std::vector<Tile> tiles;
+    b->geom = sc::RebuildGeom{};
+    b->has_vv = b->has_bd = b->has_rescale = false;
+    int run0 = 0;
+    for (int r = 0; r < b->nact; r++) {
+        Slot &sl = b->slots[b->act[r]];
+        sl.h.tile0 = (int)tiles.size();
+        sl.h.run0 = run0;
+        run0 += sl.h.nruns;
+        for (int t = 0; t < sl.h.ntiles; t++) tiles.push_back(Tile{r, t * SC_TB, t, 0});
+        reps.push_back(sl.h);
+        b->geom.max_chunks = std::max(b->geom.max_chunks, (sl.h.ncell + SC_CHUNK - 1) / SC_CHUNK);
+        if (!sl.h.lds_build) b->geom.any_tile_build = 1;
+        if (sl.h.sys == SC_SYS_BROWNIAN)
+            b->has_bd = true;
+        else
+            b->has_vv = true;
+        if (sl.h.rescale) b->has_rescale = true;
+    }
+    b->geom.nruns = run0;
+    b->ntiles = (int)tiles.size();
+    if (b->ntiles > b->tiles_cap) {
+        if (b->d_tiles) (void)hipFree(b->d_tiles);
+        b->d_tiles = nullptr;
+        int cap = std::max(b->ntiles, 2 * b->tiles_cap);
+        if (hipMalloc(&b->d_tiles, sizeof(Tile) * cap) != hipSuccess) return bfail(b, SC_ERR_OOM, "tile table");
+        b->tiles_cap = cap;
+    }
+    if (b->ntiles) BCHK(b, hipMemcpy(b->d_tiles, tiles.data(), sizeof(Tile) * b->ntiles, hipMemcpyHostToDevice));
+    if (b->nact) BCHK(b, hipMemcpy(b->d_reps, reps.data(), sizeof(Rep) * b->nact, hipMemcpyHostToDevice));
+    b->dirty = false;
+    return SC_OK;
+}
+
+int enqueue_step(sc_batch *b)
+{
+    const int nt = b->ntiles, nr = b->nact;
+    if (b->has_bd) {
+        BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 2, 4, b->s));
+        BK(b, sc::launch_drift_brownian(b->d_tiles, nt, b->d_reps, nr, b->s));
+    }
+    if (b->has_vv) BK(b, sc::launch_kick_drift(b->d_tiles, nt, b->d_reps, nr, b->s));
+    BK(b, sc::launch_rebuild(b->d_tiles, nt, b->d_reps, nr, b->geom, b->s, true));
+    if (b->has_vv) BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 1, 3, b->s));
+    if (b->has_rescale) BK(b, sc::launch_ekin_com(b->d_tiles, nt, b->d_reps, nr, b->s));
+    return SC_OK;
+}
+
+int check_slot(sc_batch *b, int id)
+{
+    if (!b || id < 0 || id >= (int)b->slots.size() || !b->slots[id].used) return SC_ERR_ARG;
+    return sync_tables(b);
+}
+
+int replica_index(sc_batch *b, int id)
+{
+    for (int r = 0; r < b->nact; r++)
+        if (b->act[r] == id) return r;
+    return -1;
+}
+
+int batch_errors(sc_batch *b)
+{
+    for (int r = 0; r < b->nact; r++) {
+        RepDyn d;
+        BCHK(b, hipMemcpyAsync(&d, b->slots[b->act[r]].h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, b->s));
+        BCHK(b, hipStreamSynchronize(b->s));
+        if (d.err & SC_EBIT_OVERFLOW)
+            return bfail(b, SC_ERR_OVERFLOW, "replica " + std::to_string(b->act[r]) + ": neighbour list overflow");
+        if (d.err & SC_EBIT_NONFINITE)
+            return bfail(b, SC_ERR_NONFINITE, "replica " + std::to_string(b->act[r]) + ": non-finite position");
+    }
+    return SC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_batch_create(sc_batch **out, int32_t max_replicas)
+{
+    if (!out || max_replicas < 1 || max_replicas > 65536) return SC_ERR_ARG;
+    sc_batch *b = new sc_batch();
+    *out = b;
+    b->max_reps = max_replicas;
+    b->slots.resize(max_replicas);
+    if (hipStreamCreate(&b->s) != hipSuccess) return bfail(b, SC_ERR_HIP, "hipStreamCreate failed");
+    BCHK(b, hipEventCreate(&b->ev0));
+    BCHK(b, hipEventCreate(&b->ev1));
+    if (hipMalloc(&b->d_reps, sizeof(Rep) * max_replicas) != hipSuccess)
+        return bfail(b, SC_ERR_OOM, "replica descriptor table");
+    return SC_OK;
+}
+
+void sc_batch_destroy(sc_batch *b)
+{
+    if (!b) return;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    drop_graph(b);
+    for (auto &sl : b->slots) sc::free_all(sl.allocs);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_reps) (void)hipFree(b->d_reps);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    delete b;
+}
+
+const char *sc_batch_last_error(const sc_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int64_t sc_replica_device_bytes(const sc_params *p, const double *D)
+{
+    if (!p || !D || !sc::params_valid(p)) return -1;
+    return sc::replica_bytes(p, D);
+}
+
+int sc_batch_add(sc_batch *b, const sc_params *p, const double *D, const double *r, const double *v, int32_t *id)
+{
+    if (!b || !p || !D || !r || !id) return SC_ERR_ARG;
+    if (!sc::params_valid(p)) return bfail(b, SC_ERR_ARG, "invalid parameters");
+    int slot = -1;
+    for (int i = 0; i < b->max_reps; i++)
+        if (!b->slots[i].used) {
+            slot = i;
+            break;
+        }
+    if (slot < 0) return bfail(b, SC_ERR_OOM, "no free replica slot");
+    Slot &sl = b->slots[slot];
+    sl.allocs.clear();
+    int rc = sc::replica_init(sl.h, p, D, sl.allocs, b->err);
+    if (rc != SC_OK) {
+        sc::free_all(sl.allocs);
+        return rc;
+    }
+    sl.p = *p;
+    sl.used = true;
+    sl.steps = 0;
+    sl.bytes = sc::replica_bytes(p, D);
+    b->dirty = true;
+    if ((rc = sync_tables(b))) return rc;
+    const Rep &h = b->slots[slot].h;
+    const size_t nd = (size_t)h.n * h.dims;
+    BCHK(b, hipMemcpyAsync(h.stage, r, nd * sizeof(double), hipMemcpyHostToDevice, b->s));
+    BK(b, sc::launch_from_host_order(b->d_tiles + h.tile0, h.ntiles, b->d_reps, FLD_R, b->s));
+    if (v) {
+        BCHK(b, hipMemcpyAsync(h.stage, v, nd * sizeof(double), hipMemcpyHostToDevice, b->s));
+        BK(b, sc::launch_from_host_order(b->d_tiles + h.tile0, h.ntiles, b->d_reps, FLD_V, b->s));
+    }
+    int one = 1;
+    BCHK(b, hipMemcpyAsync(&h.dyn->rebuild, &one, sizeof(int), hipMemcpyHostToDevice, b->s));
+    BK(b, sc::launch_rebuild(b->d_tiles, b->ntiles, b->d_reps, b->nact, b->geom, b->s, true));
+    BCHK(b, hipStreamSynchronize(b->s));
+    *id = slot;
+    return batch_errors(b);
+}
+
+int sc_batch_remove(sc_batch *b, int32_t id)
+{
+    if (!b || id < 0 || id >= (int)b->slots.size() || !b->slots[id].used) return SC_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->s));
+    sc::free_all(b->slots[id].allocs);
+    b->slots[id].used = false;
+    b->dirty = true;
+    return SC_OK;
+}
+
+int sc_batch_set_graph(sc_batch *b, int32_t on)
+{
+    if (!b) return SC_ERR_ARG;
+    b->use_graph = on != 0;
+    return SC_OK;
+}
+
+int sc_batch_step(sc_batch *b, int64_t nsteps)
+{
+    if (!b || nsteps < 0) return SC_ERR_ARG;
+    int rc = sync_tables(b);
+    if (rc) return rc;
+    if (b->nact == 0 || nsteps == 0) return SC_OK;
+    BCHK(b, hipEventRecord(b->ev0, b->s));
+    if (b->use_graph) {
+        if (!b->gexec) {
+            BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_step(b);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->s, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return bfail(b, SC_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+            b->graph = g;
+            BCHK(b, hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+        }
+        for (int64_t s = 0; s < nsteps; s++) BCHK(b, hipGraphLaunch(b->gexec, b->s));
+    } else {
+        for (int64_t s = 0; s < nsteps; s++)
+            if ((rc = enqueue_step(b))) return rc;
+    }
+    BK(b, sc::launch_finalize(b->d_tiles, b->ntiles, b->d_reps, b->nact, b->s));
+    BCHK(b, hipEventRecord(b->ev1, b->s));
+    BCHK(b, hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    BCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    for (int r = 0; r < b->nact; r++) b->slots[b->act[r]].steps += nsteps;
+    return batch_errors(b);
+}
+
+int sc_batch_minimize(sc_batch *b, int64_t iters, double dt, double max_step)
+{
+    if (!b || iters < 0 || !(dt > 0.0) || !(max_step > 0.0)) return SC_ERR_ARG;
+    int rc = sync_tables(b);
+    if (rc) return rc;
+    if (b->nact == 0) return SC_OK;
+    // pair forces only: Newtonian views of the descriptors for the descent
+    std::vector<Rep> view(b->nact);
+    for (int r = 0; r < b->nact; r++) {
+        view[r] = b->slots[b->act[r]].h;
+        view[r].sys = SC_SYS_NEWTONIAN;
+    }
+    BCHK(b, hipMemcpyAsync(b->d_reps, view.data(), sizeof(Rep) * b->nact, hipMemcpyHostToDevice, b->s));
+    const int nt = b->ntiles, nr = b->nact;
+    for (int64_t it = 0; it < iters; it++) {
+        BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 0, 7, b->s));
+        BK(b, sc::launch_min_scale(b->d_tiles, nt, b->d_reps, nr, dt, max_step, b->s));
+        BK(b, sc::launch_min_move(b->d_tiles, nt, b->d_reps, nr, dt, b->s));
+        BK(b, sc::launch_rebuild(b->d_tiles, nt, b->d_reps, nr, b->geom, b->s, true));
+    }
+    for (int r = 0; r < b->nact; r++) view[r] = b->slots[b->act[r]].h;
+    BCHK(b, hipMemcpyAsync(b->d_reps, view.data(), sizeof(Rep) * b->nact, hipMemcpyHostToDevice, b->s));
+    BCHK(b, hipStreamSynchronize(b->s));
+    return batch_errors(b);
+}
+
+int sc_batch_get_state(sc_batch *b, int32_t id, double *r, double *v, double *F)
+{
+    int rc = check_slot(b, id);
+    if (rc) return rc;
+    const Rep &h = b->slots[id].h;
+    const size_t nd = (size_t)h.n * h.dims;
+    double *dst[3] = {r, v, F};
+    const int fld[3] = {FLD_R, FLD_V, FLD_F};
+    for (int q = 0; q < 3; q++) {
+        if (!dst[q]) continue;
+        BK(b, sc::launch_to_host_order(b->d_tiles + h.tile0, h.ntiles, b->d_reps, fld[q], b->s));
+        BCHK(b, hipMemcpyAsync(dst[q], h.stage, nd * sizeof(double), hipMemcpyDeviceToHost, b->s));
+        BCHK(b, hipStreamSynchronize(b->s));
+    }
+    return SC_OK;
+}
+
+int sc_batch_set_state(sc_batch *b, int32_t id, const double *r, const double *v, const double *F)
+{
+    int rc = check_slot(b, id);
+    if (rc) return rc;
+    const Rep &h = b->slots[id].h;
+    const size_t nd = (size_t)h.n * h.dims;
+    const double *src[3] = {r, v, F};
+    const int fld[3] = {FLD_R, FLD_V, FLD_F};
+    for (int q = 0; q < 3; q++) {
+        if (!src[q]) continue;
+        BCHK(b, hipMemcpyAsync(h.stage, src[q], nd * sizeof(double), hipMemcpyHostToDevice, b->s));
+        BK(b, sc::launch_from_host_order(b->d_tiles + h.tile0, h.ntiles, b->d_reps, fld[q], b->s));
+    }
+    BCHK(b, hipStreamSynchronize(b->s));
+    return SC_OK;
+}
+
+int sc_batch_replica_stats(sc_batch *b, int32_t id, int64_t *steps, int64_t *rebuilds, double *epot)
+{
+    int rc = check_slot(b, id);
+    if (rc) return rc;
+    const Rep &h = b->slots[id].h;
+    if (epot) {
+        const int ri = replica_index(b, id);
+        if (ri < 0) return SC_ERR_ARG;
+        BK(b, sc::launch_energy(b->d_tiles + h.tile0, h.ntiles, b->d_reps, b->nact, b->s));
+    }
+    RepDyn d;
+    BCHK(b, hipMemcpyAsync(&d, h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, b->s));
+    BCHK(b, hipStreamSynchronize(b->s));
+    if (steps) *steps = b->slots[id].steps;
+    if (rebuilds) *rebuilds = d.rebuilds;
+    if (epot) *epot = d.result[0];
+    return SC_OK;
+}
+
+int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *device_bytes, double *last_ms)
+{
+    if (!b) return SC_ERR_ARG;
+    int rc = sync_tables(b);
+    if (rc) return rc;
+    int64_t np = 0, by = 0;
+    for (int r = 0; r < b->nact; r++) {
+        np += b->slots[b->act[r]].h.n;
+        by += b->slots[b->act[r]].bytes;
+    }
+    if (active) *active = b->nact;
+    if (particles) *particles = np;
+    if (device_bytes) *device_bytes = by;
+    if (last_ms) *last_ms = b->last_ms;
+    return SC_OK;
+}
+
+}  // extern "C"

@@ -122,5 +122,16 @@ int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, int nreps, 
 int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, hipStream_t s);
 }  // namespace sc
 
+// replica setup (sc_replica.hip)
+#include <string>
+#include <vector>
+struct sc_params;
+namespace sc {
+bool params_valid(const sc_params *p);
+int64_t replica_bytes(const sc_params *p, const double *D);
+int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *> &allocs, std::string &err);
+void free_all(std::vector<void *> &allocs);
+}  // namespace sc
+
 // fields for the host-order transfer kernels
 enum { FLD_R = 0, FLD_V, FLD_F, FLD_D, FLD_DISP, FLD_DISP2 };

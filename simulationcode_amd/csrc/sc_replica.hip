@@ -1,0 +1,201 @@
+// sc_replica.hip -- device state of one replica (one periodic system), shared
+// by the single-system context (sc_create) and the replica batch.
+//
+// initialize_neighbor_struct (src/DataStructs.jl:349-378) fixes the Verlet
+// radius R_v = skin + force_cutoff * max(D); the cell grid, the tile count and
+// every device buffer follow from it and from N.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sc.h"
+#include "sc_internal.h"
+
+namespace sc {
+
+bool params_valid(const sc_params *p)
+{
+    return !((p->dims != 2 && p->dims != 3) || (p->potential != SC_POT_LJ && p->potential != SC_POT_BERTHIER) ||
+             p->system < 0 || p->system > 2 || (p->precision != SC_PREC_FP64 && p->precision != SC_PREC_MIXED) ||
+             p->N < 2 || p->N > (int64_t)1 << 30 || p->maxneighbours < 1 || p->maxneighbours > 100000 ||
+             !(p->box_size > 0.0) || !(p->skin_distance >= 0.0) || !(p->force_cutoff > 0.0) ||
+             p->cell_divisor < 0 || p->cell_divisor > 4 || p->build_mode < 0 || p->build_mode > 1);
+}
+
+// Cell grid of a replica: cells of side >= R_v / k with a (2k+1)^dims
+// stencil; a hair of slack so floating-point cell assignment can never drop
+// a pair at exactly R_v.  Fewer than 2k+1 cells per side => one cell.
+static void cell_grid(const sc_params *p, double Rv, long long &nc, int &srange)
+{
+    const int k = p->cell_divisor > 0 ? p->cell_divisor : 2;
+    nc = (long long)std::floor(k * p->box_size / (Rv * (1.0 + 1e-10)));
+    srange = k;
+    if (nc < 2 * k + 1) {
+        nc = 1;
+        srange = 0;
+    }
+}
+
+int64_t replica_bytes(const sc_params *p, const double *D)
+{
+    double maxD = D[0];
+    for (int64_t i = 1; i < p->N; i++) maxD = std::max(maxD, D[i]);
+    const double Rv = p->skin_distance + p->force_cutoff * maxD;
+    long long nc;
+    int srange;
+    cell_grid(p, Rv, nc, srange);
+    const long long ncell = nc * nc * (p->dims == 3 ? nc : 1);
+    const long long n = p->N, d = p->dims;
+    const long long ntiles = (n + SC_TB - 1) / SC_TB;
+    long long b = 0;
+    b += 2 * n * 32;                  // pos, pos_alt
+    b += 4 * n * d * 8;               // v, v_alt, F, disp
+    b += 5 * n * 4;                   // orig, orig_alt, cell_of, rank, key
+    b += (2 * ncell + 1) * 4 + 4 * ((ncell + SC_CHUNK - 1) / SC_CHUNK);
+    b += n * 4 + n * p->maxneighbours * 4;  // nn, nbr
+    b += ntiles * 8 * 8 + n * 3 * 8 + 4096;  // partials, staging, dyn
+    return b;
+}
+
+static int alloc_into(std::vector<void *> &allocs, void **ptr, size_t bytes, std::string &err)
+{
+    void *q = nullptr;
+    if (bytes == 0) bytes = 1;
+    if (hipMalloc(&q, bytes) != hipSuccess) {
+        *ptr = nullptr;
+        err = "device allocation of " + std::to_string(bytes) + " bytes failed";
+        return SC_ERR_OOM;
+    }
+    allocs.push_back(q);
+    *ptr = q;
+    return SC_OK;
+}
+
+template <typename T>
+static int A(std::vector<void *> &allocs, T **ptr, size_t count, std::string &err)
+{
+    return alloc_into(allocs, reinterpret_cast<void **>(ptr), count * sizeof(T), err);
+}
+
+int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *> &allocs, std::string &err)
+{
+    std::memset(&h, 0, sizeof(Rep));
+    const int n = (int)p->N;
+    double maxD = D[0];
+    for (int i = 1; i < n; i++) maxD = std::max(maxD, D[i]);
+    // initialize_neighbor_struct, src/DataStructs.jl:351-352
+    const double Rv = p->skin_distance + p->force_cutoff * maxD;
+    if (!(p->box_size > 2.0 * Rv)) {
+        err = "box_size must exceed twice the Verlet radius";
+        return SC_ERR_ARG;
+    }
+    h.dims = p->dims;
+    h.pot = p->potential;
+    h.sys = p->system;
+    h.prec = p->precision;
+    h.n = n;
+    h.cap = (int)p->maxneighbours;
+    h.rescale = (p->system == SC_SYS_NEWTONIAN && p->rescale_temperature) ? 1 : 0;
+    h.L = p->box_size;
+    h.invL = 1.0 / p->box_size;
+    h.fc2 = p->force_cutoff2;
+    h.rcut = std::sqrt(p->force_cutoff2);  // src/NeighborLists.jl:44
+    h.skin = p->skin_distance;
+    h.Rv2 = Rv * Rv;
+    h.thr = (p->skin_distance / 2.0) * (p->skin_distance / 2.0);
+    std::memcpy(&h.thr_bits, &h.thr, sizeof(double));
+    h.eps = p->lj_eps;
+    h.lj_s2 = p->lj_sigma * p->lj_sigma;
+    h.lj_md = p->lj_sigma * p->lj_sigma;  // find_mean_D for LJ returns sigma^2
+    h.Uc = p->lj_Uc;
+    h.c0 = p->b_c0;
+    h.c2 = p->b_c2;
+    h.c4 = p->b_c4;
+    h.zeta = p->b_zeta;
+    h.kBT = p->kBT;
+    h.gamma = p->gamma;
+    h.dt = p->dt;
+    h.m = p->m;
+    h.seed = p->seed;
+    long long nc;
+    int srange;
+    cell_grid(p, Rv, nc, srange);
+    const long long ncell = nc * nc * (p->dims == 3 ? nc : 1);
+    if (ncell > (1LL << 27)) {
+        err = "cell grid too large";
+        return SC_ERR_ARG;
+    }
+    h.nc = (int)nc;
+    h.ncell = (int)ncell;
+    h.srange = srange;
+    h.cinv = (double)nc / p->box_size;
+    h.ntiles = (n + SC_TB - 1) / SC_TB;
+    // LDS build runs: ~64 own particles per block, window <= 64 cells
+    h.lds_build = (srange == 1 && p->build_mode == 1) ? 1 : 0;
+    if (h.lds_build) {
+        const double per_cell = (double)n / (double)ncell;
+        const int cxmax = (p->dims == 3 ? 5 : 19);
+        int cx = (int)std::lround(64.0 / std::max(per_cell, 1e-9));
+        cx = std::max(1, std::min(cx, std::min(cxmax, (int)nc - 2)));
+        h.cx = cx;
+        const long long rpr = (nc + cx - 1) / cx;
+        h.nruns = (int)(rpr * nc * (p->dims == 3 ? nc : 1));
+    }
+    const int d = p->dims;
+    const int nchunks = (int)((ncell + SC_CHUNK - 1) / SC_CHUNK);
+    int rc = SC_OK;
+    rc |= A(allocs, &h.pos, n, err);
+    rc |= A(allocs, &h.pos_alt, n, err);
+    rc |= A(allocs, &h.v, (size_t)n * d, err);
+    rc |= A(allocs, &h.v_alt, (size_t)n * d, err);
+    rc |= A(allocs, &h.F, (size_t)n * d, err);
+    rc |= A(allocs, &h.disp, (size_t)n * d, err);
+    rc |= A(allocs, &h.orig, n, err);
+    rc |= A(allocs, &h.orig_alt, n, err);
+    rc |= A(allocs, &h.cell_of, n, err);
+    rc |= A(allocs, &h.rank, n, err);
+    rc |= A(allocs, &h.key, n, err);
+    rc |= A(allocs, &h.chunk_off, (size_t)nchunks, err);
+    rc |= A(allocs, &h.cell_cnt, (size_t)ncell, err);
+    rc |= A(allocs, &h.cell_start, (size_t)ncell + 1, err);
+    rc |= A(allocs, &h.nn, n, err);
+    rc |= A(allocs, &h.nbr, (size_t)n * h.cap, err);
+    rc |= A(allocs, &h.part, (size_t)h.ntiles * 8, err);
+    rc |= A(allocs, &h.stage, (size_t)n * 3, err);
+    rc |= A(allocs, &h.dyn, 1, err);
+    if (rc != SC_OK) return SC_ERR_OOM;
+
+    std::vector<int> ident(n);
+    for (int i = 0; i < n; i++) ident[i] = i;
+    std::vector<double4> pos(n);
+    for (int i = 0; i < n; i++) pos[i] = make_double4(0.0, 0.0, 0.0, D[i]);
+    RepDyn dyn;
+    std::memset(&dyn, 0, sizeof(dyn));
+    dyn.vscale = 1.0;
+    bool ok = true;
+    ok &= hipMemcpy(h.orig, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemcpy(h.pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice) == hipSuccess;
+    ok &= hipMemset(h.v, 0, sizeof(double) * n * d) == hipSuccess;
+    ok &= hipMemset(h.F, 0, sizeof(double) * n * d) == hipSuccess;
+    ok &= hipMemset(h.disp, 0, sizeof(double) * n * d) == hipSuccess;
+    ok &= hipMemset(h.cell_cnt, 0, sizeof(int) * ncell) == hipSuccess;
+    ok &= hipMemset(h.nn, 0, sizeof(int) * n) == hipSuccess;
+    ok &= hipMemset(h.part, 0, sizeof(double) * h.ntiles * 8) == hipSuccess;
+    ok &= hipMemset(h.chunk_off, 0, sizeof(int) * nchunks) == hipSuccess;
+    ok &= hipMemcpy(h.dyn, &dyn, sizeof(RepDyn), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        err = "replica initialisation copy failed";
+        return SC_ERR_HIP;
+    }
+    return SC_OK;
+}
+
+void free_all(std::vector<void *> &allocs)
+{
+    for (void *q : allocs) (void)hipFree(q);
+    allocs.clear();
+}
+
+}  // namespace sc
