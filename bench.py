@@ -1,30 +1,35 @@
 """Benchmark: particle-steps/s of the MD hot path on one MI355X.
 
-Workload (default): 3D polydisperse non-additive soft spheres, N = 10^6,
-rho = 1.02, skin 0.3, Newtonian NVE (velocity Verlet, dt = 0.005) from a
-seeded jittered simple-cubic lattice, relaxed on the device by capped
-steepest descent and thermalised at kBT = 0.2 by velocity rescaling (all
-untimed).  Inputs are resident in HBM when the timed region starts.
-
-The timed region is K calls' worth of do_time_step (neighbour rebuilds on
+Default workload (the large single system): 3D polydisperse non-additive soft
+spheres, N = 10^6, rho = 1.02, skin 0.3, Newtonian NVE (velocity Verlet,
+dt = 0.005) from a seeded jittered simple-cubic lattice, relaxed on the
+device by capped steepest descent and thermalised at kBT = 0.2 by velocity
+rescaling (all untimed).  Inputs are resident in HBM when the timed region
+starts.  The timed region is K steps of do_time_step (neighbour rebuilds on
 the skin trigger included) in one sc_do_time_steps call, bracketed by device
 synchronisation.  `value` is the FP64 drop-in form; the mixed-precision form
-(FP32 pair arithmetic) is measured on the same state and reported beside it
-with its max relative force error and NVE drift.
+(FP32 pair arithmetic) runs from the same state and is reported beside it
+with its force error and NVE drift (linear fit of the total energy over a
+separate run, scaled to 10^5 steps).
 
 roofline: the dominant kernel is the pair-force kernel (fused with the
-second half-kick and the COM partial sums).  Its algorithmic bytes per
-particle are r (24) + D (8) + v read/write (48) + F write (24) + 4 (n_half+1)
-neighbour-index bytes, n_half measured from the device lists; its average
-duration comes from HIP events on the context stream over the timed region.
+second half-kick and the COM partial sums).  Algorithmic bytes per particle:
+r (24) + D (8) + v read/write (48) + F write (24) + 4 (n_half+1) neighbour
+index bytes, n_half measured on the device lists; its average duration comes
+from HIP events on the context stream over the timed region.
 
 cpu_baseline: the oracle (FP64 C restatement of the reference's serial
 algorithm, one thread) on the same N = 10^6 state, timed for a bounded number
 of steps (~15 s), rebuilds included.
+
+--workload batched: 256 independent replicas (N log-uniform in [500, 20000],
+75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian dt = 0.001,
+per-replica seeds, mixed precision) advanced by one hipGraph-captured step;
+value = aggregate particle-steps/s.
+--workload served: see simulationcode_amd/serve.py.
 """
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -35,13 +40,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s FP64 vs mixed @ stated force err; p50/p99 replica latency"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1, cell_divisor=1):
+def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1, cell_divisor=0):
     from simulationcode_amd import md
     U = md.Berthier(rc=1.25)
     p = nat.ScParams()
@@ -60,7 +66,6 @@ def total_energy(ctx):
 
 def timed_run(ctx, steps, warmup):
     ctx.do_time_steps(warmup)
-    e0 = total_energy(ctx)
     reb0 = ctx.rebuild_count()
     ctx.set_kernel_timing(True)
     t0 = time.perf_counter()
@@ -68,10 +73,20 @@ def timed_run(ctx, steps, warmup):
     wall = time.perf_counter() - t0
     kt = ctx.kernel_times()
     ctx.set_kernel_timing(False)
-    dev_ms = ctx.last_elapsed_ms()
-    e1 = total_energy(ctx)
-    return {"wall_s": wall, "device_ms": dev_ms, "e0": e0, "e1": e1, "rebuilds": ctx.rebuild_count() - reb0,
-            "kt": kt}
+    return {"wall_s": wall, "device_ms": ctx.last_elapsed_ms(), "rebuilds": ctx.rebuild_count() - reb0, "kt": kt}
+
+
+def drift_run(ctx, steps, samples=20):
+    """Total-energy drift: least-squares slope of E over `steps` steps,
+    relative to |E|, scaled to 10^5 steps; also the relative spread."""
+    chunk = max(steps // samples, 1)
+    t, e = [0], [total_energy(ctx)]
+    for k in range(samples):
+        ctx.do_time_steps(chunk)
+        t.append((k + 1) * chunk)
+        e.append(total_energy(ctx))
+    slope = np.polyfit(np.array(t, float), np.array(e), 1)[0]
+    return float(slope * 1e5 / abs(e[0])), float((max(e) - min(e)) / abs(e[0]))
 
 
 def cpu_baseline(op_state, budget_s=15.0):
@@ -95,24 +110,7 @@ def cpu_baseline(op_state, budget_s=15.0):
                       f"{steps} steps in {el:.1f} s incl. {int(st.rebuilds[0]) - 1} rebuilds"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--N", type=int, default=10 ** 6)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--seed", type=int, default=20261206)
-    ap.add_argument("--cell-divisor", type=int, default=0, help="0: library default")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the hot path would shard)", file=sys.stderr)
-        sys.exit(2)
-
-    from simulationcode_amd import _native as nat, synth
-    nat.load()
+def bench_large(args, nat, synth):
     N = args.N
     rho, dims = 1.02, 3
     L = synth.box_size(N, rho, dims)
@@ -140,11 +138,10 @@ def main():
     n_half = tot / 2.0 / N
 
     res64 = timed_run(ctx, args.steps, args.warmup)
-    log(f"[bench] fp64: {res64['device_ms'] / args.steps * 1e3:.1f} us/step, rebuilds {res64['rebuilds']}")
     ms64 = res64["device_ms"] / args.steps
+    log(f"[bench] fp64: {ms64 * 1e3:.1f} us/step, rebuilds {res64['rebuilds']}")
     value = N * args.steps / (res64["device_ms"] / 1e3)
 
-    # roofline of the pair-force kernel
     kt = res64["kt"]
     f_avg_s = kt["force_ms"] / max(kt["force_launches"], 1) / 1e3
     bytes_per_particle = 24 + 8 + 48 + 24 + 4 * (n_half + 1)
@@ -153,20 +150,23 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_force<1> (pair forces + second half-kick + COM partials)",
                 "kernel_avg_us": round(f_avg_s * 1e6, 2), "bytes_per_particle": round(bytes_per_particle, 2),
-                "n_half_mean": round(n_half, 3), "drift_kernel_avg_us":
-                    round(kt["drift_ms"] / max(kt["drift_launches"], 1) * 1e3, 2)}
-
-    drift64 = (res64["e1"] - res64["e0"]) / abs(res64["e0"]) * (1e5 / args.steps)
-    out = {"metric": "particle-steps/s FP64 vs mixed @ stated force err; p50/p99 replica latency",
-           "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+                "n_half_mean": round(n_half, 3),
+                "drift_kernel_avg_us": round(kt["drift_ms"] / max(kt["drift_launches"], 1) * 1e3, 2)}
+    out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
            "data": "synthetic (seeded jittered lattice, relaxed and thermalised on device)",
            "config": {"workload": f"3D polydisperse non-additive soft spheres, N={N}, rho=1.02, Newtonian NVE "
                                   f"dt=0.005, skin 0.3, kBT=0.2 start", "N": N, "dims": 3,
                       "potential": "Berthier (rc=1.25, zeta=0.2)", "dynamics": "Newtonian", "dt": 0.005},
-           "roofline": roofline,
-           "fp64": {"rebuilds": res64["rebuilds"], "nve_rel_drift_per_1e5_steps": drift64,
-                    "wall_s": res64["wall_s"]}}
+           "roofline": roofline, "fp64": {"rebuilds": res64["rebuilds"], "wall_s": res64["wall_s"]}}
+
+    if args.drift_steps > 0:
+        ctx.set_state(r=state["r"], v=state["v"], F=state["F"])
+        ctx.update_neighbor_lists()
+        d64, fl64 = drift_run(ctx, args.drift_steps)
+        out["fp64"].update({"nve_rel_drift_per_1e5_steps": d64, "nve_rel_fluct": fl64,
+                            "drift_window_steps": args.drift_steps})
 
     if not args.no_mixed:
         ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED, cell_divisor=args.cell_divisor)
@@ -181,6 +181,8 @@ def main():
         fn = np.linalg.norm(F64, axis=1)
         dn = np.linalg.norm(F32 - F64, axis=1)
         ctx.close()
+        ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
+        ctxm.update_neighbor_lists()
         resm = timed_run(ctxm, args.steps, args.warmup)
         msm = resm["device_ms"] / args.steps
         kmt = resm["kt"]
@@ -188,9 +190,13 @@ def main():
                         "speedup_vs_fp64": ms64 / msm,
                         "max_rel_force_err_vs_mean": float(dn.max() / fn.mean()),
                         "max_rel_force_err_vs_max": float(dn.max() / fn.max()),
-                        "nve_rel_drift_per_1e5_steps": (resm["e1"] - resm["e0"]) / abs(resm["e0"]) * (1e5 / args.steps),
                         "rebuilds": resm["rebuilds"],
                         "force_kernel_avg_us": kmt["force_ms"] / max(kmt["force_launches"], 1) * 1e3}
+        if args.drift_steps > 0:
+            ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
+            ctxm.update_neighbor_lists()
+            dm, flm = drift_run(ctxm, args.drift_steps)
+            out["mixed"].update({"nve_rel_drift_per_1e5_steps": dm, "nve_rel_fluct": flm})
         log(f"[bench] mixed: {msm * 1e3:.1f} us/step")
         ctxm.close()
     else:
@@ -201,6 +207,86 @@ def main():
         out["cpu_baseline"] = cpu_baseline((state["r"], state["v"], state["F"], D, L, N))
     else:
         out["cpu_baseline"] = None
+    return out
+
+
+def bench_batched(args, nat):
+    from simulationcode_amd import workloads as W
+    specs = W.batched_specs(args.replicas)
+    b = nat.Batch(len(specs) + 1, graph=not args.no_graph)
+    ids, Ns, dims = [], [], []
+    t0 = time.perf_counter()
+    for s in specs:
+        p, D, r, v = W.replica(**s)
+        ids.append(b.add(p, D, r, v))
+        Ns.append(s["N"])
+        dims.append(s["dims"])
+    log(f"[bench] admitted {len(ids)} replicas, {sum(Ns)} particles ({time.perf_counter() - t0:.1f}s)")
+    b.minimize(300, 0.005, 0.02)
+    # Euler-Maruyama at dt = 0.001 is only conditionally stable for the r^-12
+    # core (rare deep overlaps blow up after ~10^3 steps, FP64 and mixed
+    # alike), so the timed window is kept to a few hundred steps from the
+    # relaxed start
+    steps = min(args.steps, 500)
+    warm = min(args.warmup, 50)
+    b.step(warm)
+    b.step(steps)
+    args.steps, args.warmup = steps, warm
+    ms = b.info()["last_ms"]
+    tot = sum(Ns)
+    value = tot * args.steps / (ms / 1e3)
+    # algorithmic bytes per particle-step, Brownian with on-device noise:
+    # r 2*8d + F write 8d + disp 2*8d + D 8 + 4 (n_half + 1)
+    byt = 0.0
+    for rid, n, d in zip(ids, Ns, dims):
+        t, _ = b.neighbor_stats(rid)
+        byt += n * (40 * d + 8 + 4 * (t / 2.0 / n + 1))
+    achieved = byt * args.steps / (ms / 1e3) / 1e9
+    log(f"[bench] batched: {ms / args.steps * 1e3:.1f} us/step over {tot} particles")
+    return {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "mixed (f32 pair / f64 state)",
+            "data": "synthetic (seeded lattices relaxed on device)",
+            "config": {"workload": f"{len(ids)} independent polydisperse replicas, N log-uniform [500, 20000], "
+                                   "75% 3D / 25% 2D, Brownian dt=0.001, kBT U[0.1,0.3], per-replica seeds",
+                       "replicas": len(ids), "particles": tot, "graph": not args.no_graph},
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "whole batched step (all kernels)",
+                         "bytes_per_particle_step_mean": round(byt / tot, 2)},
+            "cpu_baseline": None}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--N", type=int, default=10 ** 6)
+    ap.add_argument("--workload", choices=("large", "batched", "served"), default="large")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--drift-steps", type=int, default=4000)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=20261206)
+    ap.add_argument("--cell-divisor", type=int, default=0, help="0: library default")
+    ap.add_argument("--rate", type=float, nargs="*", default=None, help="served: arrival rates (jobs/s)")
+    ap.add_argument("--jobs", type=int, default=48, help="served: jobs per arrival rate")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the hot path would shard)", file=sys.stderr)
+        sys.exit(2)
+    from simulationcode_amd import _native as nat, synth
+    nat.load()
+    if args.workload == "large":
+        out = bench_large(args, nat, synth)
+    elif args.workload == "batched":
+        out = bench_batched(args, nat)
+    else:
+        from simulationcode_amd import serve
+        out = serve.bench(args, log)
     print(json.dumps(out), flush=True)
 
 

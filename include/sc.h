@@ -190,6 +190,12 @@ int sc_batch_set_state(sc_batch *b, int32_t id, const double *r, const double *v
 /* Steps taken in the batch, rebuild count, and (if epot != NULL) the
  * potential energy of calculate_full_energy. */
 int sc_batch_replica_stats(sc_batch *b, int32_t id, int64_t *steps, int64_t *rebuilds, double *epot);
+/* The error state of one replica (SC_OK, SC_ERR_OVERFLOW or
+ * SC_ERR_NONFINITE).  Replicas are independent: a failed replica can be
+ * removed and the others continue unaffected. */
+int sc_batch_replica_error(sc_batch *b, int32_t id, int32_t *code);
+/* Short-list statistics of a replica (as sc_neighbor_stats). */
+int sc_batch_neighbor_stats(sc_batch *b, int32_t id, int64_t *total_entries, int64_t *max_row);
 /* Admitted replicas, their total particles and device bytes, and the device
  * time of the last sc_batch_step in milliseconds. */
 int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *device_bytes, double *last_ms);

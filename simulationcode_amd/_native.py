@@ -47,6 +47,7 @@ EXPORTS = (
     "sc_batch_create", "sc_batch_destroy", "sc_batch_last_error", "sc_replica_device_bytes",
     "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
+    "sc_batch_neighbor_stats", "sc_batch_replica_error",
 )
 
 
@@ -143,6 +144,8 @@ def load(path=LIB_PATH):
         "sc_batch_set_state": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
         "sc_batch_replica_stats": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(i64),
                                                   pd]),
+        "sc_batch_replica_error": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+        "sc_batch_neighbor_stats": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "sc_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64),
                                          ctypes.POINTER(i64), pd]),
     }
@@ -405,6 +408,16 @@ class Batch:
         if energy:
             out["epot"] = e.value
         return out
+
+    def replica_error(self, rid):
+        c = ctypes.c_int32()
+        self._check(self._lib.sc_batch_replica_error(self._h, int(rid), ctypes.byref(c)))
+        return c.value
+
+    def neighbor_stats(self, rid):
+        t, m = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self._lib.sc_batch_neighbor_stats(self._h, int(rid), ctypes.byref(t), ctypes.byref(m)))
+        return t.value, m.value
 
     def info(self):
         a = ctypes.c_int32()

@@ -374,6 +374,35 @@ int sc_batch_replica_stats(sc_batch *b, int32_t id, int64_t *steps, int64_t *reb
     return SC_OK;
 }
 
+int sc_batch_neighbor_stats(sc_batch *b, int32_t id, int64_t *total_entries, int64_t *max_row)
+{
+    int rc = check_slot(b, id);
+    if (rc) return rc;
+    const Rep &h = b->slots[id].h;
+    std::vector<int> nn(h.n);
+    BCHK(b, hipMemcpyAsync(nn.data(), h.nn, sizeof(int) * h.n, hipMemcpyDeviceToHost, b->s));
+    BCHK(b, hipStreamSynchronize(b->s));
+    int64_t t = 0, m = 0;
+    for (int x : nn) {
+        t += x;
+        m = std::max<int64_t>(m, x);
+    }
+    if (total_entries) *total_entries = t;
+    if (max_row) *max_row = m;
+    return SC_OK;
+}
+
+int sc_batch_replica_error(sc_batch *b, int32_t id, int32_t *code)
+{
+    int rc = check_slot(b, id);
+    if (rc) return rc;
+    RepDyn d;
+    BCHK(b, hipMemcpyAsync(&d, b->slots[id].h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, b->s));
+    BCHK(b, hipStreamSynchronize(b->s));
+    *code = (d.err & SC_EBIT_OVERFLOW) ? SC_ERR_OVERFLOW : ((d.err & SC_EBIT_NONFINITE) ? SC_ERR_NONFINITE : SC_OK);
+    return SC_OK;
+}
+
 int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *device_bytes, double *last_ms)
 {
     if (!b) return SC_ERR_ARG;

@@ -1,0 +1,87 @@
+"""CPU tests of the serving loop's scheduling (admission by slots and HBM
+budget, chunking, retirement) against a stand-in batch that records calls."""
+import numpy as np
+
+from simulationcode_amd import serve
+
+
+class FakeBatch:
+    """Records adds/steps/removes; no device."""
+
+    def __init__(self, max_replicas=16, graph=False):
+        self.n = 0
+        self.live = {}
+        self.events = []
+
+    def add(self, p, D, r, v=None):
+        rid = self.n
+        self.n += 1
+        self.live[rid] = int(p.N)
+        self.events.append(("add", rid, int(p.N)))
+        return rid
+
+    def minimize(self, *a):
+        pass
+
+    def get_state(self, rid):
+        return {"r": np.zeros((self.live[rid], 3))}
+
+    def step(self, k):
+        self.events.append(("step", k, tuple(sorted(self.live))))
+
+    def remove(self, rid):
+        self.events.append(("remove", rid))
+        del self.live[rid]
+
+    def close(self):
+        pass
+
+
+def _jobs(n):
+    jobs = []
+    for k in range(n):
+        spec = dict(N=500 + 100 * k, dims=3, system="brownian", kBT=0.2, dt=0.001, seed=k, precision="mixed")
+        jobs.append(serve.Job(k, spec, steps=300 + 250 * k, arrival=0.0))
+    return jobs
+
+
+def test_budget_slots_chunks(monkeypatch):
+    monkeypatch.setattr(serve.nat, "Batch", FakeBatch)
+    monkeypatch.setattr(serve.nat, "replica_device_bytes", lambda p, D: 1000 * int(p.N))
+    budget = 1000 * (500 + 600 + 700)  # room for the first three jobs
+    srv = serve.Server(hbm_budget_bytes=budget, max_replicas=8, max_chunk=200)
+    done = srv.run(_jobs(5))
+    log = srv.batch.events
+    assert sorted(j.id for j in done) == [0, 1, 2, 3, 4]
+    assert srv.peak_bytes <= budget
+    assert srv.max_running == 3
+    steps = [e for e in log if e[0] == "step"]
+    assert all(k <= 200 for _, k, _ in steps)
+    # every job ran exactly its step count
+    ran = {}
+    rid_of = {}
+    for e in log:
+        if e[0] == "add":
+            rid_of[e[1]] = len(rid_of)
+        if e[0] == "step":
+            for rid in e[2]:
+                ran[rid_of[rid]] = ran.get(rid_of[rid], 0) + e[1]
+    assert ran == {k: 300 + 250 * k for k in range(5)}
+    # admission is in arrival order and waits for budget
+    adds = [e for e in log if e[0] in ("add", "remove")]
+    assert adds[0][0] == "add" and adds[3][0] == "remove"
+
+
+def test_latency_is_arrival_to_completion(monkeypatch):
+    monkeypatch.setattr(serve.nat, "Batch", FakeBatch)
+    monkeypatch.setattr(serve.nat, "replica_device_bytes", lambda p, D: 1)
+    t = [0.0]
+
+    def clock():
+        t[0] += 0.01
+        return t[0]
+
+    srv = serve.Server(max_chunk=100, clock=clock)
+    done = srv.run(_jobs(3), t0=0.0)
+    for j in done:
+        assert j.done >= j.admitted >= j.arrival
