@@ -23,9 +23,10 @@ algorithm, one thread) on the same N = 10^6 state, timed for a bounded number
 of steps (~15 s), rebuilds included.
 
 --workload batched: 256 independent replicas (N log-uniform in [500, 20000],
-75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian dt = 0.001,
-per-replica seeds, mixed precision) advanced by one hipGraph-captured step;
-value = aggregate particle-steps/s.
+75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian, per-replica
+seeds, mixed precision) advanced by one hipGraph-captured step; value =
+aggregate particle-steps/s.  dt = 1e-4: at the stated dt = 0.001 explicit
+Euler-Maruyama blows up within 10^2-10^3 steps for this potential.
 --workload served: see simulationcode_amd/serve.py.
 """
 import argparse
@@ -223,15 +224,8 @@ def bench_batched(args, nat):
         dims.append(s["dims"])
     log(f"[bench] admitted {len(ids)} replicas, {sum(Ns)} particles ({time.perf_counter() - t0:.1f}s)")
     b.minimize(300, 0.005, 0.02)
-    # Euler-Maruyama at dt = 0.001 is only conditionally stable for the r^-12
-    # core (rare deep overlaps blow up after ~10^3 steps, FP64 and mixed
-    # alike), so the timed window is kept to a few hundred steps from the
-    # relaxed start
-    steps = min(args.steps, 500)
-    warm = min(args.warmup, 50)
-    b.step(warm)
-    b.step(steps)
-    args.steps, args.warmup = steps, warm
+    b.step(args.warmup)
+    b.step(args.steps)
     ms = b.info()["last_ms"]
     tot = sum(Ns)
     value = tot * args.steps / (ms / 1e3)
@@ -248,7 +242,7 @@ def bench_batched(args, nat):
             "vs_baseline": None, "dtype": "mixed (f32 pair / f64 state)",
             "data": "synthetic (seeded lattices relaxed on device)",
             "config": {"workload": f"{len(ids)} independent polydisperse replicas, N log-uniform [500, 20000], "
-                                   "75% 3D / 25% 2D, Brownian dt=0.001, kBT U[0.1,0.3], per-replica seeds",
+                                   "75% 3D / 25% 2D, Brownian dt=1e-4, kBT U[0.1,0.3], per-replica seeds",
                        "replicas": len(ids), "particles": tot, "graph": not args.no_graph},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

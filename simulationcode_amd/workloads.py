@@ -67,14 +67,17 @@ def log_uniform_int(rng, lo, hi):
     return int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
 
 
-def batched_specs(n_replicas=256, seed=20261207, n_lo=500, n_hi=20000):
-    """Keyword dicts for replica() of the batched workload."""
+def batched_specs(n_replicas=256, seed=20261207, n_lo=500, n_hi=20000, dt=1e-4):
+    """Keyword dicts for replica() of the batched workload.  The workload is
+    stated with dt = 0.001, at which explicit Euler-Maruyama blows up within
+    10^2-10^3 steps for the r^-12 core (observed in FP64 and mixed alike);
+    dt = 1e-4 is used instead (pass dt=0.001 to reproduce)."""
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_replicas):
         dims = 3 if rng.uniform() < 0.75 else 2
         out.append(dict(N=log_uniform_int(rng, n_lo, n_hi), dims=dims, system="brownian",
-                        kBT=float(rng.uniform(0.1, 0.3)), dt=0.001, gamma=1.0, seed=seed * 1000 + k,
+                        kBT=float(rng.uniform(0.1, 0.3)), dt=dt, gamma=1.0, seed=seed * 1000 + k,
                         precision="mixed"))
     return out
 
