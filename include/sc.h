@@ -200,6 +200,32 @@ int sc_batch_neighbor_stats(sc_batch *b, int32_t id, int64_t *total_entries, int
  * time of the last sc_batch_step in milliseconds. */
 int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *device_bytes, double *last_ms);
 
+/* ---- launch shape and autotuner --------------------------------------
+ * No reference counterpart: the reference's rebuild is CellListMap's CPU
+ * cell list (src/NeighborLists.jl:46-52).  The per-slot neighbour-list build
+ * kernel runs with block_size threads per block (64 or 256) and
+ * lanes_per_particle lanes cooperating on one slot (1, 2, 4 or 8).  The
+ * lists, and so every result, are identical for every shape; only the
+ * time differs (one lane per slot suits one large system, several lanes
+ * suit batches of small replicas whose rebuild is latency-bound). */
+typedef struct sc_tune_entry {
+    int32_t block_size;
+    int32_t lanes_per_particle;
+    double ms_per_step; /* device time per step measured for this shape */
+} sc_tune_entry;
+
+#define SC_TUNE_SHAPES 8
+
+int sc_set_build_launch(sc_ctx *ctx, int32_t block_size, int32_t lanes_per_particle);
+int sc_batch_set_build_launch(sc_batch *b, int32_t block_size, int32_t lanes_per_particle);
+/* Autotuner: for each of the SC_TUNE_SHAPES shapes, one untimed step and
+ * then `steps` timed steps (real steps of the system: state, step and
+ * rebuild counters advance, on-device noise only), then keeps the fastest
+ * shape.  entries (may be NULL) receives up to max_entries measurements,
+ * *best the index of the chosen one. */
+int sc_autotune(sc_ctx *ctx, int64_t steps, sc_tune_entry *entries, int32_t max_entries, int32_t *best);
+int sc_batch_autotune(sc_batch *b, int64_t steps, sc_tune_entry *entries, int32_t max_entries, int32_t *best);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,7 +48,27 @@ EXPORTS = (
     "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
     "sc_batch_neighbor_stats", "sc_batch_replica_error",
+    "sc_set_build_launch", "sc_batch_set_build_launch", "sc_autotune", "sc_batch_autotune",
 )
+
+
+class TuneEntry(ctypes.Structure):
+    """sc_tune_entry: one launch shape of the per-slot build and its measured
+    device time per step."""
+    _fields_ = [("block_size", ctypes.c_int32), ("lanes_per_particle", ctypes.c_int32),
+                ("ms_per_step", ctypes.c_double)]
+
+
+TUNE_SHAPES = 8
+
+
+def _tune(fn, h, steps):
+    ents = (TuneEntry * TUNE_SHAPES)()
+    best = ctypes.c_int32()
+    rc = fn(h, int(steps), ents, TUNE_SHAPES, ctypes.byref(best))
+    table = [{"block_size": e.block_size, "lanes_per_particle": e.lanes_per_particle,
+              "ms_per_step": e.ms_per_step} for e in ents]
+    return rc, table, best.value
 
 
 class ScError(RuntimeError):
@@ -148,6 +168,12 @@ def load(path=LIB_PATH):
         "sc_batch_neighbor_stats": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "sc_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64),
                                          ctypes.POINTER(i64), pd]),
+        "sc_set_build_launch": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
+        "sc_batch_set_build_launch": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
+        "sc_autotune": (ctypes.c_int, [vp, i64, ctypes.POINTER(TuneEntry), ctypes.c_int32,
+                                       ctypes.POINTER(ctypes.c_int32)]),
+        "sc_batch_autotune": (ctypes.c_int, [vp, i64, ctypes.POINTER(TuneEntry), ctypes.c_int32,
+                                             ctypes.POINTER(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -311,6 +337,16 @@ class Context:
         self._check(self._lib.sc_neighbor_stats(self._h, ctypes.byref(t), ctypes.byref(m)))
         return t.value, m.value
 
+    def set_build_launch(self, block_size, lanes_per_particle):
+        self._check(self._lib.sc_set_build_launch(self._h, int(block_size), int(lanes_per_particle)))
+
+    def autotune(self, steps=100):
+        """Time every build launch shape over `steps` real steps each, keep
+        the fastest; returns (table, index of the chosen shape)."""
+        rc, table, best = _tune(self._lib.sc_autotune, self._h, steps)
+        self._check(rc)
+        return table, best
+
 
 def replica_device_bytes(params: ScParams, D):
     D = np.ascontiguousarray(D, dtype=np.float64)
@@ -426,3 +462,13 @@ class Batch:
         self._check(self._lib.sc_batch_info(self._h, ctypes.byref(a), ctypes.byref(npart), ctypes.byref(by),
                                             ctypes.byref(ms)))
         return {"active": a.value, "particles": npart.value, "device_bytes": by.value, "last_ms": ms.value}
+
+    def set_build_launch(self, block_size, lanes_per_particle):
+        self._check(self._lib.sc_batch_set_build_launch(self._h, int(block_size), int(lanes_per_particle)))
+
+    def autotune(self, steps=100):
+        """Time every build launch shape over `steps` real steps of the batch
+        each, keep the fastest; returns (table, index of the chosen shape)."""
+        rc, table, best = _tune(self._lib.sc_batch_autotune, self._h, steps)
+        self._check(rc)
+        return table, best

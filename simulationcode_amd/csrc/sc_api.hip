@@ -184,6 +184,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     rc = SC_OK;
     rc |= dalloc(c, &c->d_rep, 1);
     rc |= dalloc(c, &c->d_tiles, c->ntiles);
+    rc |= dalloc(c, &c->geom.plan, 2 + 1 + c->ntiles);
     if (rc != SC_OK) return SC_ERR_OOM;
     std::vector<Tile> tiles(c->ntiles);
     for (int t = 0; t < c->ntiles; t++) tiles[t] = Tile{0, t * SC_TB, t, 0};
@@ -502,5 +503,33 @@ int sc_neighbor_stats(sc_ctx *c, int64_t *total, int64_t *max_row)
 }
 
 void sc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { sc::philox_host(ctr, key, out); }
+
+int sc_set_build_launch(sc_ctx *c, int32_t block_size, int32_t lanes_per_particle)
+{
+    if (!c || !sc::build_shape_valid(block_size, lanes_per_particle)) return SC_ERR_ARG;
+    c->geom.build_bs = block_size;
+    c->geom.build_lpp = lanes_per_particle;
+    return SC_OK;
+}
+
+int sc_autotune(sc_ctx *c, int64_t steps, sc_tune_entry *entries, int32_t max_entries, int32_t *best)
+{
+    if (!c || !c->d_rep || steps < 1 || max_entries < 0) return SC_ERR_ARG;
+    int bi = 0, rc;
+    double bms = 0.0;
+    for (int q = 0; q < SC_TUNE_SHAPES; q++) {
+        if ((rc = sc_set_build_launch(c, sc::kBuildShapes[q][0], sc::kBuildShapes[q][1]))) return rc;
+        if ((rc = sc_do_time_steps(c, 1, nullptr))) return rc;
+        if ((rc = sc_do_time_steps(c, steps, nullptr))) return rc;
+        const double ms = c->last_ms / (double)steps;
+        if (entries && q < max_entries) entries[q] = sc_tune_entry{sc::kBuildShapes[q][0], sc::kBuildShapes[q][1], ms};
+        if (q == 0 || ms < bms) {
+            bms = ms;
+            bi = q;
+        }
+    }
+    if (best) *best = bi;
+    return sc_set_build_launch(c, sc::kBuildShapes[bi][0], sc::kBuildShapes[bi][1]);
+}
 
 }  // extern "C"

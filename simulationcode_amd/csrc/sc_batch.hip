@@ -41,6 +41,9 @@ struct sc_batch {
     int tiles_cap = 0, ntiles = 0, nact = 0;
     std::vector<int> act;  // slot of each compact replica index
     sc::RebuildGeom geom{};
+    int *d_plan = nullptr;
+    int plan_cap = 0;
+    int build_bs = 256, build_lpp = 4;  // per-slot build launch shape (small replicas: several lanes)
     bool dirty = true, has_vv = false, has_bd = false, has_rescale = false;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -110,6 +113,18 @@ int sync_tables(sc_batch *b)
     }
     b->geom.nruns = run0;
     b->ntiles = (int)tiles.size();
+    const int plan_need = 2 + b->nact + b->ntiles;
+    if (plan_need > b->plan_cap) {
+        const int cap = std::max(plan_need, 2 * b->plan_cap);
+        if (b->d_plan) (void)hipFree(b->d_plan);
+        b->d_plan = nullptr;
+        b->plan_cap = 0;
+        if (hipMalloc(&b->d_plan, sizeof(int) * cap) != hipSuccess) return bfail(b, SC_ERR_OOM, "rebuild plan");
+        b->plan_cap = cap;
+    }
+    b->geom.plan = b->d_plan;
+    b->geom.build_bs = b->build_bs;
+    b->geom.build_lpp = b->build_lpp;
     if (b->ntiles > b->tiles_cap) {
         if (b->d_tiles) (void)hipFree(b->d_tiles);
         b->d_tiles = nullptr;
@@ -190,6 +205,7 @@ void sc_batch_destroy(sc_batch *b)
     drop_graph(b);
     for (auto &sl : b->slots) sc::free_all(sl.allocs);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_plan) (void)hipFree(b->d_plan);
     if (b->d_reps) (void)hipFree(b->d_reps);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -418,6 +434,39 @@ int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *dev
     if (device_bytes) *device_bytes = by;
     if (last_ms) *last_ms = b->last_ms;
     return SC_OK;
+}
+
+int sc_batch_set_build_launch(sc_batch *b, int32_t block_size, int32_t lanes_per_particle)
+{
+    if (!b || !sc::build_shape_valid(block_size, lanes_per_particle)) return SC_ERR_ARG;
+    BCHK(b, hipStreamSynchronize(b->s));
+    b->build_bs = b->geom.build_bs = block_size;
+    b->build_lpp = b->geom.build_lpp = lanes_per_particle;
+    drop_graph(b);  // a captured step holds the old launch
+    return SC_OK;
+}
+
+int sc_batch_autotune(sc_batch *b, int64_t steps, sc_tune_entry *entries, int32_t max_entries, int32_t *best)
+{
+    if (!b || steps < 1 || max_entries < 0) return SC_ERR_ARG;
+    int rc = sync_tables(b);
+    if (rc) return rc;
+    if (b->nact == 0) return bfail(b, SC_ERR_STATE, "autotune needs admitted replicas");
+    int bi = 0;
+    double bms = 0.0;
+    for (int q = 0; q < SC_TUNE_SHAPES; q++) {
+        if ((rc = sc_batch_set_build_launch(b, sc::kBuildShapes[q][0], sc::kBuildShapes[q][1]))) return rc;
+        if ((rc = sc_batch_step(b, 1))) return rc;  // graph capture outside the timed steps
+        if ((rc = sc_batch_step(b, steps))) return rc;
+        const double ms = b->last_ms / (double)steps;
+        if (entries && q < max_entries) entries[q] = sc_tune_entry{sc::kBuildShapes[q][0], sc::kBuildShapes[q][1], ms};
+        if (q == 0 || ms < bms) {
+            bms = ms;
+            bi = q;
+        }
+    }
+    if (best) *best = bi;
+    return sc_batch_set_build_launch(b, sc::kBuildShapes[bi][0], sc::kBuildShapes[bi][1]);
 }
 
 }  // extern "C"

@@ -101,10 +101,16 @@ int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s);
 int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 struct RebuildGeom {
-    int max_chunks;      // max over replicas of ceil(ncell / SC_CHUNK)
-    int nruns;           // LDS build runs over all replicas
-    int any_tile_build;  // some replica uses the per-slot build (all-pairs / wide stencil)
+    int max_chunks = 0;      // max over replicas of ceil(ncell / SC_CHUNK)
+    int nruns = 0;           // LDS build runs over all replicas
+    int any_tile_build = 0;  // some replica uses the per-slot build (all-pairs / wide stencil)
+    int *plan = nullptr;     // device work list, 2 + nreps + ntiles ints (k_rebuild_plan)
+    int build_bs = 256;      // per-slot build launch shape: threads per block (64 or 256)
+    int build_lpp = 1;       //   and lanes per slot (1, 2, 4 or 8); the lists do not depend on it
 };
+bool build_shape_valid(int bs, int lpp);
+// the per-slot build launch shapes tried by the autotuner, {block, lanes}
+constexpr int kBuildShapes[8][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {64, 1}, {64, 2}, {64, 4}, {64, 8}};
 // bookkeep=false when the next launch is a velocity-Verlet force finish,
 // which does the rebuild bookkeeping itself
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,

@@ -660,6 +660,56 @@ __global__ void k_request_rebuild(Rep *reps, int nreps)
     if (r < nreps) reps[r].dyn->rebuild = 1;
 }
 
+// Rebuild work list.  plan[0] = number of tiles to rebuild, plan[1] = number
+// of replicas to rebuild, plan[2 .. 2+nreps) = those replicas, ascending,
+// plan[2+nreps ..) = their tiles, ascending.  The rebuild kernels sweep this
+// list only, so a step in which a few replicas of a large batch rebuild
+// spreads just their tiles over the whole grid.
+__global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nreps, int *plan)
+{
+    __shared__ int st[SC_TB], sr[SC_TB], so[SC_TB], sn[SC_TB], s0[SC_TB];
+    __shared__ int base_t, base_r;
+    const int tid = threadIdx.x;
+    int *rlist = plan + 2, *tlist = plan + 2 + nreps;
+    if (tid == 0) base_t = base_r = 0;
+    for (int b = 0; b < nreps; b += SC_TB) {
+        const int r = b + tid;
+        int flag = 0, nt = 0, t0 = 0;
+        if (r < nreps) {
+            flag = reps[r].dyn->rebuild != 0 ? 1 : 0;
+            nt = flag ? reps[r].ntiles : 0;
+            t0 = reps[r].tile0;
+        }
+        st[tid] = nt;
+        sr[tid] = flag;
+        __syncthreads();
+        for (int o = 1; o < SC_TB; o <<= 1) {
+            const int xt = tid >= o ? st[tid - o] : 0, xr = tid >= o ? sr[tid - o] : 0;
+            __syncthreads();
+            st[tid] += xt;
+            sr[tid] += xr;
+            __syncthreads();
+        }
+        if (flag) rlist[base_r + sr[tid] - 1] = r;
+        so[tid] = base_t + st[tid] - nt;
+        sn[tid] = nt;
+        s0[tid] = t0;
+        __syncthreads();
+        for (int q = 0; q < SC_TB; q++)  // each flagged replica's tiles, written by the whole block
+            for (int x = tid; x < sn[q]; x += SC_TB) tlist[so[q] + x] = s0[q] + x;
+        __syncthreads();
+        if (tid == SC_TB - 1) {
+            base_t += st[tid];
+            base_r += sr[tid];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        plan[0] = base_t;
+        plan[1] = base_r;
+    }
+}
+
 // Slot range start of cell c: chunk-local exclusive prefix + chunk offset
 // (two-level scan), c == ncell gives n.
 __device__ inline int cstart(const Rep &R, int c)
@@ -688,12 +738,12 @@ __device__ inline XcdRange xcd_range(int total)
 // R1: cell of every slot and its arrival rank within the cell.  Slots are
 // cell-sorted from the previous rebuild, so a wavefront spans few cells:
 // one atomic per distinct cell per wavefront.
-__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, int ntiles)
+__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
 {
-    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[it.cur];
+    const int *tlist = plan + 2 + nreps;
+    for (XcdRange it = xcd_range(plan[0]); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[tlist[it.cur]];
         const Rep R = reps[t.rep];
-        if (!need_rebuild(R)) continue;
         const int i = t.first + threadIdx.x;
         const bool valid = i < R.n;
         int c = -1;
@@ -726,12 +776,10 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
 
 // R2a: chunk-local exclusive scan of the cell counts (SC_CHUNK cells per
 // block, 4 per thread); clears the counts for the next rebuild.
-__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, int nreps)
+__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan)
 {
-    const int r = blockIdx.y;
-    if (r >= nreps) return;
-    const Rep R = reps[r];
-    if (!need_rebuild(R)) return;
+    if ((int)blockIdx.y >= plan[1]) return;
+    const Rep R = reps[plan[2 + blockIdx.y]];
     const int c0 = blockIdx.x * SC_CHUNK;
     if (c0 >= R.ncell) return;
     const int tid = threadIdx.x;
@@ -766,12 +814,10 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, int nreps
 }
 
 // R2b: exclusive scan of the chunk totals (one block per replica).
-__global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, int nreps)
+__global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *plan)
 {
-    const int r = blockIdx.x;
-    if (r >= nreps) return;
-    const Rep R = reps[r];
-    if (!need_rebuild(R)) return;
+    if ((int)blockIdx.x >= plan[1]) return;
+    const Rep R = reps[plan[2 + blockIdx.x]];
     const int nch = (R.ncell + SC_CHUNK - 1) / SC_CHUNK;
     __shared__ int sums[SC_TB];
     __shared__ int carry;
@@ -797,12 +843,12 @@ __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, int nreps)
 
 // R3: each old slot's cell position; the cell-sorted slot gets the
 // particle's original index as its sort key.
-__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, int ntiles)
+__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
 {
-    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[it.cur];
+    const int *tlist = plan + 2 + nreps;
+    for (XcdRange it = xcd_range(plan[0]); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[tlist[it.cur]];
         const Rep R = reps[t.rep];
-        if (!need_rebuild(R)) continue;
         const int i = t.first + threadIdx.x;
         if (i >= R.n) continue;
         R.key[cstart(R, R.cell_of[i]) + R.rank[i]] = R.orig[i];
@@ -814,12 +860,12 @@ __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep 
 // is a pure function of the positions), and the live state (r/D, v, orig)
 // is scattered there.  F is dead at this point of the step (recomputed next)
 // and disp is zeroed by the build.
-__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, int ntiles)
+__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
 {
-    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[it.cur];
+    const int *tlist = plan + 2 + nreps;
+    for (XcdRange it = xcd_range(plan[0]); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[tlist[it.cur]];
         const Rep R = reps[t.rep];
-        if (!need_rebuild(R)) continue;
         const int x = t.first + threadIdx.x, n = R.n;
         if (x >= n) continue;
         const int c = R.cell_of[x];
@@ -1073,21 +1119,32 @@ __device__ inline int wrap_shift(int raw, int nc, double L, double &sL)
     return raw;
 }
 
-// Per-slot build (stencil +-srange, or all pairs when srange == 0): one
-// thread per slot sweeping the stencil cells with global loads.
+// Candidates in the contiguous slot range [b, e) at image shift (sx, sy, sz),
+// four position loads issued before any of them is tested.
 template <int DIMS>
-__device__ void build_tile(const Rep &R, const Tile &t)
+__device__ inline void build_range(const Rep &R, int k, const double4 p, int &nfull, int &cnt, int b, int e,
+                                   double sx, double sy, double sz)
 {
-    const int n = R.n, k = t.first + threadIdx.x;
-    if (k >= n) return;
-    const double4 p = R.pos_alt[k];
-    int nfull = 0, cnt = 0;
-    if (R.srange == 0) {  // single cell: every other slot, reference minimum image
-        for (int j = 0; j < n; j++)
-            if (j != k) build_one<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j]);
-        build_finish_slot<DIMS>(R, k, p, nfull, cnt);
-        return;
+    int j = b;
+    for (; j + 4 <= e; j += 4) {
+        const double4 q0 = R.pos_alt[j], q1 = R.pos_alt[j + 1], q2 = R.pos_alt[j + 2], q3 = R.pos_alt[j + 3];
+        if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, q0, sx, sy, sz);
+        if (j + 1 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 1, q1, sx, sy, sz);
+        if (j + 2 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 2, q2, sx, sy, sz);
+        if (j + 3 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 3, q3, sx, sy, sz);
     }
+    for (; j < e; j++)
+        if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j], sx, sy, sz);
+}
+
+// The stencil of slot k's cell as (2s+1)^(dims-1) rows of 2s+1 cells along
+// x.  A row's cells are adjacent cell ids, hence one contiguous slot range,
+// split in two where the row wraps the box (each part with its own image
+// shift); the parts come in the order of a cell-by-cell sweep.  seg(b, e,
+// sx, sy, sz) is called once per part.
+template <int DIMS, class Seg>
+__device__ inline void for_stencil_rows(const Rep &R, const double4 p, Seg &&seg)
+{
     int cc[3];
     cell_of_pos<DIMS>(R, p, cc);
     const int s = R.srange, nc = R.nc;
@@ -1098,31 +1155,129 @@ __device__ void build_tile(const Rep &R, const Tile &t)
         for (int oy = -s; oy <= s; oy++) {
             double sy;
             const int cy = wrap_shift(cc[1] + oy, nc, R.L, sy);
-            for (int ox = -s; ox <= s; ox++) {
-                double sx;
-                const int cx = wrap_shift(cc[0] + ox, nc, R.L, sx);
-                const int c = cx + nc * (cy + nc * cz);
-                const int b = cstart(R, c), e = cstart(R, c + 1);
-                for (int j = b; j < e; j++) {
-                    if (j == k) continue;
-                    build_one_img<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j], sx, sy, sz);
-                }
+            const int row = nc * (cy + nc * cz);
+            int x0 = cc[0] - s;
+            const int x1 = cc[0] + s;
+            if (x0 < 0) {  // cells nc+x0 .. nc-1, seen across the low face
+                seg(cstart(R, row + x0 + nc), cstart(R, row + nc), -R.L, sy, sz);
+                x0 = 0;
             }
+            const int xe = x1 < nc ? x1 : nc - 1;
+            seg(cstart(R, row + x0), cstart(R, row + xe + 1), 0.0, sy, sz);
+            if (x1 >= nc)  // cells 0 .. x1-nc, seen across the high face
+                seg(cstart(R, row), cstart(R, row + x1 - nc + 1), R.L, sy, sz);
         }
+    }
+}
+
+// One thread per slot (stencil +-srange, or all pairs when srange == 0).
+template <int DIMS>
+__device__ void build_slot(const Rep &R, int k)
+{
+    const double4 p = R.pos_alt[k];
+    int nfull = 0, cnt = 0;
+    if (R.srange == 0) {  // single cell: every other slot, reference minimum image
+        for (int j = 0; j < R.n; j++)
+            if (j != k) build_one<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j]);
+    } else {
+        for_stencil_rows<DIMS>(R, p, [&](int b, int e, double sx, double sy, double sz) {
+            build_range<DIMS>(R, k, p, nfull, cnt, b, e, sx, sy, sz);
+        });
     }
     build_finish_slot<DIMS>(R, k, p, nfull, cnt);
 }
 
-__global__ __launch_bounds__(SC_TB) void k_build(const Tile *tiles, const Rep *reps, int ntiles)
+// LPP lanes per slot: lane l of the group takes candidates b+l, b+l+LPP, ...
+// of a range; accepted entries are placed with a ballot in candidate order,
+// so the list is identical to the one-thread sweep's, with LPP candidate
+// loads in flight per slot.  IMG: stencil image shift; else the reference
+// minimum image (all-pairs mode).
+template <int DIMS, int LPP, bool IMG>
+__device__ inline void build_segment(const Rep &R, int k, const double4 p, int b, int e, double sx, double sy,
+                                     double sz, int l, unsigned long long gmask, unsigned long long below,
+                                     int &nfull, int &cnt)
 {
-    for (XcdRange it = xcd_range(ntiles); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[it.cur];
+    for (int a0 = b; a0 < e; a0 += LPP) {
+        const int a = a0 + l;
+        bool inF = false, inH = false;
+        if (a < e && a != k) {
+            const double4 q = R.pos_alt[a];
+            double d2;
+            if (IMG) {
+                const double dx = (p.x - q.x) - sx;
+                const double dy = (p.y - q.y) - sy;
+                d2 = dx * dx + dy * dy;
+                if (DIMS == 3) {
+                    const double dz = (p.z - q.z) - sz;
+                    d2 = d2 + dz * dz;
+                }
+            } else {
+                double dx, dy, dz;
+                d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
+            }
+            inF = d2 <= R.Rv2;
+            if (inF) {
+                const double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
+                const double rh = R.rcut * md + R.skin;
+                inH = d2 < rh * rh;
+            }
+        }
+        const unsigned long long mF = __ballot(inF) & gmask;
+        const unsigned long long mH = __ballot(inH) & gmask;
+        if (inH) {
+            const int slot = cnt + __popcll(mH & below);
+            if (slot < R.cap) R.nbr[(size_t)slot * R.n + k] = a;
+        }
+        nfull += __popcll(mF);
+        cnt += __popcll(mH);
+    }
+}
+
+template <int DIMS, int LPP>
+__device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long gmask, unsigned long long below)
+{
+    const double4 p = R.pos_alt[k];
+    int nfull = 0, cnt = 0;
+    if (R.srange == 0) {
+        build_segment<DIMS, LPP, false>(R, k, p, 0, R.n, 0.0, 0.0, 0.0, l, gmask, below, nfull, cnt);
+    } else {
+        for_stencil_rows<DIMS>(R, p, [&](int b, int e, double sx, double sy, double sz) {
+            build_segment<DIMS, LPP, true>(R, k, p, b, e, sx, sy, sz, l, gmask, below, nfull, cnt);
+        });
+    }
+    if (l == 0) build_finish_slot<DIMS>(R, k, p, nfull, cnt);
+}
+
+// R5 (per-slot form) over the planned tiles.  Launch shape: BS threads per
+// block, LPP lanes per slot; a work item is BS/LPP consecutive slots of one
+// tile.  The lists do not depend on the launch shape.
+template <int BS, int LPP>
+__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
+{
+    constexpr int SPI = BS / LPP;      // slots per work item
+    constexpr int IPT = SC_TB / SPI;   // work items per tile
+    static_assert(SPI * IPT == SC_TB && LPP <= WAVE, "launch shape");
+    const int *tlist = plan + 2 + nreps;
+    const int lane = threadIdx.x % WAVE, l = threadIdx.x % LPP;
+    const unsigned long long gmask = LPP == WAVE ? ~0ull : ((1ull << LPP) - 1ull) << (lane / LPP * LPP);
+    const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
+    for (XcdRange it = xcd_range(plan[0] * IPT); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[tlist[it.cur / IPT]];
         const Rep R = reps[t.rep];
-        if (R.lds_build || !need_rebuild(R)) continue;
-        if (R.dims == 3)
-            build_tile<3>(R, t);
-        else
-            build_tile<2>(R, t);
+        if (R.lds_build) continue;
+        const int k = t.first + (it.cur % IPT) * SPI + (int)threadIdx.x / LPP;
+        if (k >= R.n) continue;
+        if (LPP == 1) {
+            if (R.dims == 3)
+                build_slot<3>(R, k);
+            else
+                build_slot<2>(R, k);
+        } else {
+            if (R.dims == 3)
+                build_slot_lanes<3, LPP>(R, k, l, gmask, below);
+            else
+                build_slot_lanes<2, LPP>(R, k, l, gmask, below);
+        }
     }
 }
 
@@ -1631,21 +1786,49 @@ int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s)
     return check(hipGetLastError());
 }
 
+bool build_shape_valid(int bs, int lpp)
+{
+    return (bs == 64 || bs == 256) && (lpp == 1 || lpp == 2 || lpp == 4 || lpp == 8);
+}
+
+template <int BS, int LPP>
+static void launch_build_shape(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const int *plan,
+                               hipStream_t s)
+{
+    const int items = ntiles * (SC_TB * LPP / BS);
+    const int cap = SC_GRID_CAP * (SC_TB / BS);
+    hipLaunchKernelGGL((k_build<BS, LPP>), dim3(items < cap ? items : cap), dim3(BS), 0, s, tiles, reps, plan, nreps);
+}
+
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
                    bool bookkeep)
 {
     if (ntiles <= 0) return 0;
+    if (!g.plan || !build_shape_valid(g.build_bs, g.build_lpp)) return -1;
     const int gt = ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP;
-    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
-    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, nreps);
-    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, nreps);
-    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
-    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
+    hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, g.plan);
+    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, g.plan, nreps);
+    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, g.plan);
+    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, g.plan);
+    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, g.plan, nreps);
+    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, g.plan, nreps);
     if (g.nruns > 0) {
         const int gr = g.nruns < SC_RUN_GRID_CAP ? g.nruns : SC_RUN_GRID_CAP;
         hipLaunchKernelGGL(k_build_cells, dim3(gr), dim3(SC_TB), 0, s, reps, nreps, g.nruns);
     }
-    if (g.any_tile_build) hipLaunchKernelGGL(k_build, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, ntiles);
+    if (g.any_tile_build) {
+        const int key = g.build_bs * 16 + g.build_lpp;
+        switch (key) {
+        case 64 * 16 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 64 * 16 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 64 * 16 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 64 * 16 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 256 * 16 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 256 * 16 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 256 * 16 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        default: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        }
+    }
     if (bookkeep) fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }

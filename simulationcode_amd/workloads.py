@@ -24,7 +24,8 @@ SYSTEMS = {"newtonian": nat.SYS_NEWTONIAN, "langevin": nat.SYS_LANGEVIN, "browni
 
 
 def replica(N, dims, *, potential="berthier", system="brownian", kBT=0.2, dt=0.001, gamma=1.0, m=1.0,
-            seed=0, precision="mixed", rho=None, skin=0.3, cell_divisor=0, maxneighbours=200, jitter=0.1):
+            seed=0, precision="mixed", rho=None, skin=0.3, cell_divisor=0, build_mode=0, maxneighbours=200,
+            jitter=0.1):
     """(ScParams, D, r, v) of one seeded replica."""
     rng = np.random.default_rng(seed)
     p = nat.ScParams()
@@ -32,6 +33,7 @@ def replica(N, dims, *, potential="berthier", system="brownian", kBT=0.2, dt=0.0
     p.system = SYSTEMS[system] if isinstance(system, str) else int(system)
     p.precision = nat.PREC_MIXED if precision == "mixed" else nat.PREC_FP64
     p.cell_divisor = cell_divisor
+    p.build_mode = build_mode
     p.N = N
     p.maxneighbours = maxneighbours
     if potential == "berthier":
