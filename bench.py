@@ -214,11 +214,19 @@ def bench_large(args, nat, synth):
 def bench_batched(args, nat):
     from simulationcode_amd import workloads as W
     specs = W.batched_specs(args.replicas)
+    t0 = time.perf_counter()
+    if args.cell_divisor:
+        cd, shape, tune = args.cell_divisor, (256, 4), None
+    else:
+        cd, shape, tune = W.autotune_batch(specs, pilot=64, steps=60)
+    log(f"[bench] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
+        f"({time.perf_counter() - t0:.1f}s)")
     b = nat.Batch(len(specs) + 1, graph=not args.no_graph)
+    b.set_build_launch(*shape)
     ids, Ns, dims = [], [], []
     t0 = time.perf_counter()
     for s in specs:
-        p, D, r, v = W.replica(**s)
+        p, D, r, v = W.replica(**dict(s, cell_divisor=cd))
         ids.append(b.add(p, D, r, v))
         Ns.append(s["N"])
         dims.append(s["dims"])
@@ -243,7 +251,9 @@ def bench_batched(args, nat):
             "data": "synthetic (seeded lattices relaxed on device)",
             "config": {"workload": f"{len(ids)} independent polydisperse replicas, N log-uniform [500, 20000], "
                                    "75% 3D / 25% 2D, Brownian dt=1e-4, kBT U[0.1,0.3], per-replica seeds",
-                       "replicas": len(ids), "particles": tot, "graph": not args.no_graph},
+                       "replicas": len(ids), "particles": tot, "graph": not args.no_graph,
+                       "cell_divisor": cd, "build_block": shape[0], "build_lanes_per_slot": shape[1],
+                       "autotune": tune},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                          "kernel": "whole batched step (all kernels)",

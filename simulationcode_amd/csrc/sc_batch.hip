@@ -43,6 +43,7 @@ struct sc_batch {
     sc::RebuildGeom geom{};
     int *d_plan = nullptr;
     int plan_cap = 0;
+    int *d_err = nullptr;  // error bits of the active replicas (max_reps ints)
     int build_bs = 256, build_lpp = 4;  // per-slot build launch shape (small replicas: several lanes)
     bool dirty = true, has_vv = false, has_bd = false, has_rescale = false;
     bool use_graph = false;
@@ -167,13 +168,15 @@ int replica_index(sc_batch *b, int id)
 
 int batch_errors(sc_batch *b)
 {
+    if (b->nact == 0) return SC_OK;
+    std::vector<int> err(b->nact);
+    BK(b, sc::launch_gather_err(b->d_reps, b->nact, b->d_err, b->s));
+    BCHK(b, hipMemcpyAsync(err.data(), b->d_err, sizeof(int) * b->nact, hipMemcpyDeviceToHost, b->s));
+    BCHK(b, hipStreamSynchronize(b->s));
     for (int r = 0; r < b->nact; r++) {
-        RepDyn d;
-        BCHK(b, hipMemcpyAsync(&d, b->slots[b->act[r]].h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, b->s));
-        BCHK(b, hipStreamSynchronize(b->s));
-        if (d.err & SC_EBIT_OVERFLOW)
+        if (err[r] & SC_EBIT_OVERFLOW)
             return bfail(b, SC_ERR_OVERFLOW, "replica " + std::to_string(b->act[r]) + ": neighbour list overflow");
-        if (d.err & SC_EBIT_NONFINITE)
+        if (err[r] & SC_EBIT_NONFINITE)
             return bfail(b, SC_ERR_NONFINITE, "replica " + std::to_string(b->act[r]) + ": non-finite position");
     }
     return SC_OK;
@@ -195,6 +198,8 @@ int sc_batch_create(sc_batch **out, int32_t max_replicas)
     BCHK(b, hipEventCreate(&b->ev1));
     if (hipMalloc(&b->d_reps, sizeof(Rep) * max_replicas) != hipSuccess)
         return bfail(b, SC_ERR_OOM, "replica descriptor table");
+    if (hipMalloc(&b->d_err, sizeof(int) * max_replicas) != hipSuccess)
+        return bfail(b, SC_ERR_OOM, "replica error array");
     return SC_OK;
 }
 
@@ -206,6 +211,7 @@ void sc_batch_destroy(sc_batch *b)
     for (auto &sl : b->slots) sc::free_all(sl.allocs);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     if (b->d_plan) (void)hipFree(b->d_plan);
+    if (b->d_err) (void)hipFree(b->d_err);
     if (b->d_reps) (void)hipFree(b->d_reps);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);

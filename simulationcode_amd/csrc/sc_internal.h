@@ -116,6 +116,7 @@ constexpr int kBuildShapes[8][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {64,
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
                    bool bookkeep = true);
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
+int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s);
 int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);

@@ -167,36 +167,53 @@ __device__ inline void normals2(uint32_t orig, long long step, uint32_t sub, uin
 // half-integer, i.e. |dx| ~ L/2, where the wrapped distance is ~L/2 either
 // way -- beyond every list radius and cutoff (L > 2 R_v is enforced), so no
 // predicate or force can change.
+// The shift n*L is exact (n is -1, 0 or 1 for wrapped positions), so the fused
+// form rounds once, exactly like dx - t.
 __device__ inline double min_image(double dx, double L, double invL)
 {
-    double n = rint(dx * invL);
-    double t = n * L;
-    return dx - t;
+    const double n = rint(dx * invL);
+    return fma(-n, L, dx);
 }
 
-// FP64 force/r of one pair, in the operation order of the reference loops:
+// FP64 reciprocal: hardware estimate refined by two Newton-Raphson steps
+// r <- r + r(1 - x r) (quadratic convergence; result within ~1 ulp of 1/x
+// for the normal, finite x met here), in place of a correctly rounded
+// division.
+__device__ inline double rcp64(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// FP64 force/r of one pair, the formulas of the reference loops:
 // Berthier src/MD.jl:120-126 (3D) / :228-234 (2D); LJ 3D :175-180; generic
-// (2D LJ) :299-307 with force() of src/InteractionPotential.jl:145-152.
+// (2D LJ) :299-307 with force() of src/InteractionPotential.jl:145-152.  The
+// cutoff tests are evaluated exactly as there; divisions are reciprocals
+// (rcp64), which moves the force by a few ulp.
 template <int DIMS, int POT>
 __device__ inline double pair_fdivr64(const Rep &R, double rij2, double Di, double Dj)
 {
     if (POT == 2) {
         double md = (Di + Dj) * 0.5 * (1.0 - R.zeta * fabs(Di - Dj));
         double md2 = md * md;
-        double inv_md2 = 1.0 / md2;
+        double inv_md2 = rcp64(md2);
         double invxi2 = rij2 * inv_md2;
         double invxi4 = invxi2 * invxi2;
-        double xi14 = 1.0 / (invxi4 * invxi4 * invxi4 * invxi2);
+        double xi14 = rcp64(invxi4 * invxi4 * invxi4 * invxi2);
         return (R.fc2 * md2 < rij2) ? 0.0 : -2.0 * inv_md2 * (R.c2 + 2.0 * R.c4 * invxi2 - 6.0 * xi14);
     } else if (DIMS == 3) {
-        double xi2 = R.lj_s2 / rij2;
+        double inv_r2 = rcp64(rij2);
+        double xi2 = R.lj_s2 * inv_r2;
         double xi6 = xi2 * xi2 * xi2;
         double xi12 = xi6 * xi6;
-        return (R.fc2 < rij2) ? 0.0 : R.eps * (48.0 * xi12 - 24.0 * xi6) / rij2;
+        return (R.fc2 < rij2) ? 0.0 : R.eps * (48.0 * xi12 - 24.0 * xi6) * inv_r2;
     } else {
         double md2 = R.lj_md * R.lj_md;
         if (R.fc2 * md2 > rij2) {
-            double inv_r2 = 1.0 / rij2;
+            double inv_r2 = rcp64(rij2);
             double xi2 = md2 * inv_r2;
             double xi6 = xi2 * xi2 * xi2;
             double xi12 = xi6 * xi6;
@@ -660,6 +677,13 @@ __global__ void k_request_rebuild(Rep *reps, int nreps)
     if (r < nreps) reps[r].dyn->rebuild = 1;
 }
 
+// Error bits of every replica into one array (one host copy per check).
+__global__ void k_gather_err(const Rep *reps, int nreps, int *out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nreps) out[r] = reps[r].dyn->err;
+}
+
 // Rebuild work list.  plan[0] = number of tiles to rebuild, plan[1] = number
 // of replicas to rebuild, plan[2 .. 2+nreps) = those replicas, ascending,
 // plan[2+nreps ..) = their tiles, ascending.  The rebuild kernels sweep this
@@ -667,7 +691,7 @@ __global__ void k_request_rebuild(Rep *reps, int nreps)
 // spreads just their tiles over the whole grid.
 __global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nreps, int *plan)
 {
-    __shared__ int st[SC_TB], sr[SC_TB], so[SC_TB], sn[SC_TB], s0[SC_TB];
+    __shared__ int st[SC_TB], sr[SC_TB], s0[SC_TB];
     __shared__ int base_t, base_r;
     const int tid = threadIdx.x;
     int *rlist = plan + 2, *tlist = plan + 2 + nreps;
@@ -676,14 +700,17 @@ __global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nre
         const int r = b + tid;
         int flag = 0, nt = 0, t0 = 0;
         if (r < nreps) {
-            flag = reps[r].dyn->rebuild != 0 ? 1 : 0;
-            nt = flag ? reps[r].ntiles : 0;
-            t0 = reps[r].tile0;
+            const Rep &R = reps[r];
+            t0 = R.tile0;
+            nt = R.ntiles;
+            flag = R.dyn->rebuild != 0 ? 1 : 0;
+            nt = flag ? nt : 0;
         }
         st[tid] = nt;
         sr[tid] = flag;
+        s0[tid] = t0;
         __syncthreads();
-        for (int o = 1; o < SC_TB; o <<= 1) {
+        for (int o = 1; o < SC_TB; o <<= 1) {  // inclusive scans of tile and replica counts
             const int xt = tid >= o ? st[tid - o] : 0, xr = tid >= o ? sr[tid - o] : 0;
             __syncthreads();
             st[tid] += xt;
@@ -691,16 +718,23 @@ __global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nre
             __syncthreads();
         }
         if (flag) rlist[base_r + sr[tid] - 1] = r;
-        so[tid] = base_t + st[tid] - nt;
-        sn[tid] = nt;
-        s0[tid] = t0;
+        const int total = st[SC_TB - 1];
+        for (int x = tid; x < total; x += SC_TB) {  // x-th planned tile of this chunk
+            int lo = 0, hi = SC_TB - 1;               // first q with st[q] > x
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (st[mid] > x)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            const int before = lo ? st[lo - 1] : 0;
+            tlist[base_t + x] = s0[lo] + (x - before);
+        }
         __syncthreads();
-        for (int q = 0; q < SC_TB; q++)  // each flagged replica's tiles, written by the whole block
-            for (int x = tid; x < sn[q]; x += SC_TB) tlist[so[q] + x] = s0[q] + x;
-        __syncthreads();
-        if (tid == SC_TB - 1) {
-            base_t += st[tid];
-            base_r += sr[tid];
+        if (tid == 0) {
+            base_t += total;
+            base_r += sr[SC_TB - 1];
         }
         __syncthreads();
     }
@@ -757,20 +791,27 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
             c = cc[0] + R.nc * (cc[1] + R.nc * cc[2]);
             R.cell_of[i] = c;
         }
+        // group the wavefront's lanes by cell (no memory traffic), then every
+        // group leader reserves its group's ranks with one atomic, all
+        // leaders at once
         const int lane = threadIdx.x % WAVE;
         unsigned long long active = __ballot(valid);
-        int rank = 0;
+        int leader_of = lane, local = 0, group = 0;
         while (active) {
             const int leader = __ffsll((long long)active) - 1;
             const int lc = __shfl(c, leader, WAVE);
             const unsigned long long same = __ballot(valid && c == lc) & active;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&R.cell_cnt[lc], __popcll(same));
-            base = __shfl(base, leader, WAVE);
-            if ((same >> lane) & 1ull) rank = base + __popcll(same & ((1ull << lane) - 1ull));
+            if ((same >> lane) & 1ull) {
+                leader_of = leader;
+                local = __popcll(same & ((1ull << lane) - 1ull));
+            }
+            if (lane == leader) group = __popcll(same);
             active &= ~same;
         }
-        if (valid) R.rank[i] = rank;
+        int base = 0;
+        if (group) base = atomicAdd(&R.cell_cnt[c], group);
+        base = __shfl(base, leader_of, WAVE);
+        if (valid) R.rank[i] = base + local;
     }
 }
 
@@ -1783,6 +1824,13 @@ int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, h
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s)
 {
     hipLaunchKernelGGL(k_request_rebuild, dim3((nreps + 255) / 256), dim3(256), 0, s, const_cast<Rep *>(reps), nreps);
+    return check(hipGetLastError());
+}
+
+int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s)
+{
+    if (nreps <= 0) return 0;
+    hipLaunchKernelGGL(k_gather_err, dim3((nreps + SC_TB - 1) / SC_TB), dim3(SC_TB), 0, s, reps, nreps, out);
     return check(hipGetLastError());
 }
 

@@ -44,11 +44,13 @@ class Job:
 
 class Server:
     def __init__(self, hbm_budget_bytes=64 << 30, max_replicas=512, max_chunk=1000, graph=True,
-                 keep_results=False, clock=time.perf_counter):
+                 keep_results=False, clock=time.perf_counter, cell_divisor=0, build_shape=(256, 4)):
         self.budget = int(hbm_budget_bytes)
         self.max_replicas = int(max_replicas)
         self.max_chunk = int(max_chunk)
+        self.cell_divisor = int(cell_divisor)  # applied to every job alike (it sets the slot order)
         self.batch = nat.Batch(self.max_replicas, graph=graph)
+        self.batch.set_build_launch(*build_shape)  # launch shape: no effect on results
         self.keep_results = keep_results
         self.clock = clock
         self.used_bytes = 0
@@ -61,7 +63,7 @@ class Server:
 
     def _bytes(self, job):
         if not hasattr(job, "_bytes"):
-            p, D, r, v = W.replica(**job.spec)
+            p, D, r, v = W.replica(**dict(job.spec, cell_divisor=self.cell_divisor))
             job._prep = (p, D, r, v)
             job._bytes = nat.replica_device_bytes(p, D)
         return job._bytes
@@ -153,9 +155,9 @@ class Server:
         self.batch.close()
 
 
-def run_alone(spec, steps, max_chunk=1000):
+def run_alone(spec, steps, max_chunk=1000, cell_divisor=0):
     """A job served on its own (reference for the alone-vs-mix check)."""
-    srv = Server(max_replicas=2, max_chunk=max_chunk, keep_results=True)
+    srv = Server(max_replicas=2, max_chunk=max_chunk, keep_results=True, cell_divisor=cell_divisor)
     job = Job(0, spec, steps, 0.0)
     srv.run([job])
     srv.close()
@@ -177,10 +179,17 @@ def bench(args, log):
     """Sweep arrival rates; per rate report p50/p99 completion latency
     (arrival to result) and aggregate particle-steps/s over the makespan."""
     rates = args.rate or [2.0, 8.0]
+    # autotuner on a pilot of the job stream's replicas (cell size, launch shape)
+    rng = np.random.default_rng(args.seed)
+    pilot = [W.served_job(rng, k)[0] for k in range(24)]
+    t0 = time.perf_counter()
+    cd, shape, _ = W.autotune_batch(pilot, pilot=24, steps=40)
+    log(f"[serve] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
+        f"({time.perf_counter() - t0:.1f}s)")
     sweep = []
     for lam in rates:
         jobs = make_jobs(args.jobs, lam, seed=args.seed + int(lam * 1000))
-        srv = Server(max_chunk=1000)
+        srv = Server(max_chunk=1000, cell_divisor=cd, build_shape=shape)
         t0 = time.perf_counter()
         done = srv.run(jobs, t0=t0)
         makespan = time.perf_counter() - t0
@@ -201,6 +210,7 @@ def bench(args, log):
             "dtype": "mixed (f32 pair / f64 state)", "data": "synthetic Poisson job stream",
             "config": {"workload": "served stream: Poisson arrivals, N log-uniform [1e3, 2e5], dims 2/3, "
                                    "Newtonian/Langevin/Brownian, steps log-uniform [1e3, 1e5]",
-                       "jobs_per_rate": args.jobs},
+                       "jobs_per_rate": args.jobs, "cell_divisor": cd, "build_block": shape[0],
+                       "build_lanes_per_slot": shape[1]},
             "p50_latency_s": best["p50_latency_s"], "p99_latency_s": best["p99_latency_s"], "sweep": sweep,
             "roofline": None, "cpu_baseline": None}

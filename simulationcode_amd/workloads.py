@@ -84,6 +84,37 @@ def batched_specs(n_replicas=256, seed=20261207, n_lo=500, n_hi=20000, dt=1e-4):
     return out
 
 
+def autotune_batch(specs, cell_divisors=(1, 2, 3), pilot=64, steps=100, relax=(300, 0.005, 0.02)):
+    """Workload-level autotuner: cell size, then launch shape.
+
+    For each candidate cell divisor (cells of side >= R_v / k, stencil +-k)
+    a pilot batch of the first `pilot` replicas of `specs` is relaxed and
+    stepped, the library's launch-shape autotuner (block size x lanes per
+    slot, sc_batch_autotune) runs on it, and the best shape's time per
+    particle-step is recorded.  Returns (best cell divisor, best shape
+    (block_size, lanes_per_particle), table).  The cell divisor changes the
+    slot order and so the bits of a replica's trajectory: it is applied to
+    every replica of a run (alone or in a mix) alike.  The launch shape
+    does not change any result."""
+    sub = specs[:pilot]
+    table = []
+    for cd in cell_divisors:
+        b = nat.Batch(len(sub) + 1, graph=True)
+        npart = 0
+        for s in sub:
+            p, D, r, v = replica(**dict(s, cell_divisor=cd))
+            b.add(p, D, r, v)
+            npart += s["N"]
+        b.minimize(*relax)
+        b.step(20)
+        shapes, best = b.autotune(steps)
+        b.close()
+        for e in shapes:
+            table.append(dict(e, cell_divisor=cd, ns_per_particle_step=e["ms_per_step"] * 1e6 / npart))
+    best = min(table, key=lambda e: e["ms_per_step"])
+    return best["cell_divisor"], (best["block_size"], best["lanes_per_particle"]), table
+
+
 def served_job(rng, job_id, n_lo=1000, n_hi=200000, steps_lo=1000, steps_hi=100000):
     """One served job: replica() keywords plus its step count."""
     dims = 2 if rng.uniform() < 0.5 else 3

@@ -23,6 +23,9 @@ class FakeBatch:
     def minimize(self, *a):
         pass
 
+    def set_build_launch(self, block_size, lanes):
+        self.events.append(("shape", block_size, lanes))
+
     def get_state(self, rid):
         return {"r": np.zeros((self.live[rid], 3))}
 
