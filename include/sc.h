@@ -182,6 +182,12 @@ int sc_batch_remove(sc_batch *b, int32_t id);
 int sc_batch_set_graph(sc_batch *b, int32_t on);
 /* Advance every admitted replica by nsteps steps. */
 int sc_batch_step(sc_batch *b, int64_t nsteps);
+/* sc_batch_step in two halves: enqueue the steps on the batch's stream and
+ * return at once (several batches then run concurrently on their own
+ * streams), and wait for them (device time, step counters, error check).
+ * Any other call on the batch waits first. */
+int sc_batch_step_async(sc_batch *b, int64_t nsteps);
+int sc_batch_wait(sc_batch *b);
 /* Capped steepest descent (sc_minimize) of every admitted replica. */
 int sc_batch_minimize(sc_batch *b, int64_t iters, double dt, double max_step);
 int sc_batch_get_state(sc_batch *b, int32_t id, double *r, double *v, double *F);

@@ -5,6 +5,7 @@ The library is built in-tree (``simulationcode_amd/_lib/libsc.so``) by
 missing or fails to load, every entry point raises.
 """
 import ctypes
+import time
 import os
 
 import numpy as np
@@ -49,6 +50,7 @@ EXPORTS = (
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
     "sc_batch_neighbor_stats", "sc_batch_replica_error",
     "sc_set_build_launch", "sc_batch_set_build_launch", "sc_autotune", "sc_batch_autotune",
+    "sc_batch_step_async", "sc_batch_wait",
 )
 
 
@@ -168,6 +170,8 @@ def load(path=LIB_PATH):
         "sc_batch_neighbor_stats": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
         "sc_batch_info": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(i64),
                                          ctypes.POINTER(i64), pd]),
+        "sc_batch_step_async": (ctypes.c_int, [vp, i64]),
+        "sc_batch_wait": (ctypes.c_int, [vp]),
         "sc_set_build_launch": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "sc_batch_set_build_launch": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "sc_autotune": (ctypes.c_int, [vp, i64, ctypes.POINTER(TuneEntry), ctypes.c_int32,
@@ -466,9 +470,92 @@ class Batch:
     def set_build_launch(self, block_size, lanes_per_particle):
         self._check(self._lib.sc_batch_set_build_launch(self._h, int(block_size), int(lanes_per_particle)))
 
+    def step_async(self, nsteps):
+        """Enqueue nsteps steps on this batch's stream and return at once."""
+        self._check(self._lib.sc_batch_step_async(self._h, int(nsteps)))
+
+    def wait(self):
+        """Wait for step_async's steps (updates timing, counters, errors)."""
+        self._check(self._lib.sc_batch_wait(self._h))
+
     def autotune(self, steps=100):
         """Time every build launch shape over `steps` real steps of the batch
         each, keep the fastest; returns (table, index of the chosen shape)."""
         rc, table, best = _tune(self._lib.sc_batch_autotune, self._h, steps)
         self._check(rc)
         return table, best
+
+
+class BatchGroup:
+    """Replicas spread over `streams` batches, each with its own HIP stream,
+    stepped concurrently: the kernels of one batch fill the gaps left by the
+    latency-bound launches of the others.  A replica goes to the batch with
+    the fewest particles; its trajectory does not depend on the grouping."""
+
+    def __init__(self, max_replicas=256, streams=2, graph=True):
+        self.batches = [Batch(max_replicas, graph=graph) for _ in range(int(streams))]
+        self.load = [0] * len(self.batches)
+        self.where = {}
+        self._next = 0
+        self.last_ms = 0.0
+
+    def add(self, params, D, r, v=None):
+        k = min(range(len(self.batches)), key=lambda q: self.load[q])
+        rid = self.batches[k].add(params, D, r, v)
+        gid = self._next
+        self._next += 1
+        self.where[gid] = (k, rid)
+        self.load[k] += int(params.N)
+        return gid
+
+    def remove(self, gid):
+        k, rid = self.where.pop(gid)
+        n = self.batches[k].info()["particles"]
+        self.batches[k].remove(rid)
+        self.load[k] -= n - self.batches[k].info()["particles"]
+
+    def _each(self, name, *a):
+        for b in self.batches:
+            getattr(b, name)(*a)
+
+    def set_build_launch(self, block_size, lanes):
+        self._each("set_build_launch", block_size, lanes)
+
+    def minimize(self, iters, dt, max_step):
+        self._each("minimize", iters, dt, max_step)
+
+    def step(self, nsteps):
+        """Steps every batch concurrently; last_ms is the wall time of the
+        slowest (events on each stream bracket its own work)."""
+        t0 = time.perf_counter()
+        live = [b for b in self.batches if b.info()["active"]]
+        for b in live:
+            b.step_async(nsteps)
+        for b in live:
+            b.wait()
+        self.last_ms = (time.perf_counter() - t0) * 1e3
+
+    def get_state(self, gid):
+        k, rid = self.where[gid]
+        return self.batches[k].get_state(rid)
+
+    def stats(self, gid, energy=False):
+        k, rid = self.where[gid]
+        return self.batches[k].stats(rid, energy)
+
+    def replica_error(self, gid):
+        k, rid = self.where[gid]
+        return self.batches[k].replica_error(rid)
+
+    def neighbor_stats(self, gid):
+        k, rid = self.where[gid]
+        return self.batches[k].neighbor_stats(rid)
+
+    def info(self):
+        infos = [b.info() for b in self.batches]
+        return {"active": sum(i["active"] for i in infos), "particles": sum(i["particles"] for i in infos),
+                "device_bytes": sum(i["device_bytes"] for i in infos), "last_ms": self.last_ms}
+
+    def close(self):
+        self._each("close")
+

@@ -47,6 +47,7 @@ struct sc_batch {
     int build_bs = 256, build_lpp = 4;  // per-slot build launch shape (small replicas: several lanes)
     bool dirty = true, has_vv = false, has_bd = false, has_rescale = false;
     bool use_graph = false;
+    int64_t pending = -1;  // steps enqueued by sc_batch_step_async and not yet waited for
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::string err;
@@ -85,6 +86,10 @@ void drop_graph(sc_batch *b)
 // or retirements.
 int sync_tables(sc_batch *b)
 {
+    if (b->pending >= 0) {  // settle an asynchronous step first
+        int rc = sc_batch_wait(b);
+        if (rc) return rc;
+    }
     if (!b->dirty) return SC_OK;
     BCHK(b, hipStreamSynchronize(b->s));
     drop_graph(b);
@@ -269,6 +274,10 @@ int sc_batch_add(sc_batch *b, const sc_params *p, const double *D, const double 
 
 int sc_batch_remove(sc_batch *b, int32_t id)
 {
+    if (b && b->pending >= 0) {
+        int rc = sc_batch_wait(b);
+        if (rc) return rc;
+    }
     if (!b || id < 0 || id >= (int)b->slots.size() || !b->slots[id].used) return SC_ERR_ARG;
     BCHK(b, hipStreamSynchronize(b->s));
     sc::free_all(b->slots[id].allocs);
@@ -279,17 +288,26 @@ int sc_batch_remove(sc_batch *b, int32_t id)
 
 int sc_batch_set_graph(sc_batch *b, int32_t on)
 {
+    if (b && b->pending >= 0) {
+        int rc = sc_batch_wait(b);
+        if (rc) return rc;
+    }
     if (!b) return SC_ERR_ARG;
     b->use_graph = on != 0;
     return SC_OK;
 }
 
-int sc_batch_step(sc_batch *b, int64_t nsteps)
+int sc_batch_step_async(sc_batch *b, int64_t nsteps)
 {
     if (!b || nsteps < 0) return SC_ERR_ARG;
+    if (b->pending >= 0) return bfail(b, SC_ERR_STATE, "a step is already in flight (call sc_batch_wait)");
     int rc = sync_tables(b);
     if (rc) return rc;
-    if (b->nact == 0 || nsteps == 0) return SC_OK;
+    if (b->nact == 0 || nsteps == 0) {
+        b->pending = 0;
+        b->last_ms = 0.0;
+        return SC_OK;
+    }
     BCHK(b, hipEventRecord(b->ev0, b->s));
     if (b->use_graph) {
         if (!b->gexec) {
@@ -309,12 +327,30 @@ int sc_batch_step(sc_batch *b, int64_t nsteps)
     }
     BK(b, sc::launch_finalize(b->d_tiles, b->ntiles, b->d_reps, b->nact, b->s));
     BCHK(b, hipEventRecord(b->ev1, b->s));
+    b->pending = nsteps;
+    return SC_OK;
+}
+
+int sc_batch_wait(sc_batch *b)
+{
+    if (!b) return SC_ERR_ARG;
+    if (b->pending < 0) return SC_OK;
+    const int64_t n = b->pending;
+    b->pending = -1;
+    if (n == 0) return SC_OK;
     BCHK(b, hipEventSynchronize(b->ev1));
     float ms = 0.f;
     BCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
-    for (int r = 0; r < b->nact; r++) b->slots[b->act[r]].steps += nsteps;
+    for (int r = 0; r < b->nact; r++) b->slots[b->act[r]].steps += n;
     return batch_errors(b);
+}
+
+int sc_batch_step(sc_batch *b, int64_t nsteps)
+{
+    int rc = sc_batch_step_async(b, nsteps);
+    if (rc) return rc;
+    return sc_batch_wait(b);
 }
 
 int sc_batch_minimize(sc_batch *b, int64_t iters, double dt, double max_step)
@@ -444,6 +480,10 @@ int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *dev
 
 int sc_batch_set_build_launch(sc_batch *b, int32_t block_size, int32_t lanes_per_particle)
 {
+    if (b && b->pending >= 0) {
+        int rc = sc_batch_wait(b);
+        if (rc) return rc;
+    }
     if (!b || !sc::build_shape_valid(block_size, lanes_per_particle)) return SC_ERR_ARG;
     BCHK(b, hipStreamSynchronize(b->s));
     b->build_bs = b->geom.build_bs = block_size;

@@ -744,6 +744,27 @@ __global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nre
     }
 }
 
+// The tiles a rebuild kernel sweeps: the planned list (batches), or, with
+// no plan (a single replica), all of the replica's tiles when its flag is set.
+struct RebuildWork {
+    const int *tl;
+    int count;
+    __device__ int tile(int w) const { return tl ? tl[w] : w; }
+};
+
+__device__ inline RebuildWork rebuild_work(const int *plan, int nreps, const Rep *reps, int ntiles)
+{
+    if (plan) return RebuildWork{plan + 2 + nreps, plan[0]};
+    return RebuildWork{nullptr, reps[0].dyn->rebuild != 0 ? ntiles : 0};
+}
+
+// Replica of the idx-th rebuilding replica, or -1.
+__device__ inline int rebuild_rep(const int *plan, const Rep *reps, int idx)
+{
+    if (plan) return idx < plan[1] ? plan[2 + idx] : -1;
+    return (idx == 0 && reps[0].dyn->rebuild != 0) ? 0 : -1;
+}
+
 // Slot range start of cell c: chunk-local exclusive prefix + chunk offset
 // (two-level scan), c == ncell gives n.
 __device__ inline int cstart(const Rep &R, int c)
@@ -772,11 +793,12 @@ __device__ inline XcdRange xcd_range(int total)
 // R1: cell of every slot and its arrival rank within the cell.  Slots are
 // cell-sorted from the previous rebuild, so a wavefront spans few cells:
 // one atomic per distinct cell per wavefront.
-__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
+__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
+                                               int ntiles)
 {
-    const int *tlist = plan + 2 + nreps;
-    for (XcdRange it = xcd_range(plan[0]); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[tlist[it.cur]];
+    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
         const int i = t.first + threadIdx.x;
         const bool valid = i < R.n;
@@ -819,8 +841,9 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
 // block, 4 per thread); clears the counts for the next rebuild.
 __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan)
 {
-    if ((int)blockIdx.y >= plan[1]) return;
-    const Rep R = reps[plan[2 + blockIdx.y]];
+    const int r = rebuild_rep(plan, reps, (int)blockIdx.y);
+    if (r < 0) return;
+    const Rep R = reps[r];
     const int c0 = blockIdx.x * SC_CHUNK;
     if (c0 >= R.ncell) return;
     const int tid = threadIdx.x;
@@ -857,8 +880,9 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
 // R2b: exclusive scan of the chunk totals (one block per replica).
 __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *plan)
 {
-    if ((int)blockIdx.x >= plan[1]) return;
-    const Rep R = reps[plan[2 + blockIdx.x]];
+    const int r = rebuild_rep(plan, reps, (int)blockIdx.x);
+    if (r < 0) return;
+    const Rep R = reps[r];
     const int nch = (R.ncell + SC_CHUNK - 1) / SC_CHUNK;
     __shared__ int sums[SC_TB];
     __shared__ int carry;
@@ -884,11 +908,12 @@ __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *
 
 // R3: each old slot's cell position; the cell-sorted slot gets the
 // particle's original index as its sort key.
-__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
+__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
+                                               int ntiles)
 {
-    const int *tlist = plan + 2 + nreps;
-    for (XcdRange it = xcd_range(plan[0]); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[tlist[it.cur]];
+    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
         const int i = t.first + threadIdx.x;
         if (i >= R.n) continue;
@@ -901,11 +926,12 @@ __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep 
 // is a pure function of the positions), and the live state (r/D, v, orig)
 // is scattered there.  F is dead at this point of the step (recomputed next)
 // and disp is zeroed by the build.
-__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
+__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
+                                               int ntiles)
 {
-    const int *tlist = plan + 2 + nreps;
-    for (XcdRange it = xcd_range(plan[0]); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[tlist[it.cur]];
+    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
         const int x = t.first + threadIdx.x, n = R.n;
         if (x >= n) continue;
@@ -1293,17 +1319,18 @@ __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long 
 // block, LPP lanes per slot; a work item is BS/LPP consecutive slots of one
 // tile.  The lists do not depend on the launch shape.
 template <int BS, int LPP>
-__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, int nreps)
+__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
+                                              int ntiles)
 {
     constexpr int SPI = BS / LPP;      // slots per work item
     constexpr int IPT = SC_TB / SPI;   // work items per tile
     static_assert(SPI * IPT == SC_TB && LPP <= WAVE, "launch shape");
-    const int *tlist = plan + 2 + nreps;
+    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
     const int lane = threadIdx.x % WAVE, l = threadIdx.x % LPP;
     const unsigned long long gmask = LPP == WAVE ? ~0ull : ((1ull << LPP) - 1ull) << (lane / LPP * LPP);
     const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
-    for (XcdRange it = xcd_range(plan[0] * IPT); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[tlist[it.cur / IPT]];
+    for (XcdRange it = xcd_range(wk.count * IPT); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[wk.tile(it.cur / IPT)];
         const Rep R = reps[t.rep];
         if (R.lds_build) continue;
         const int k = t.first + (it.cur % IPT) * SPI + (int)threadIdx.x / LPP;
@@ -1845,21 +1872,25 @@ static void launch_build_shape(const Tile *tiles, int ntiles, const Rep *reps, i
 {
     const int items = ntiles * (SC_TB * LPP / BS);
     const int cap = SC_GRID_CAP * (SC_TB / BS);
-    hipLaunchKernelGGL((k_build<BS, LPP>), dim3(items < cap ? items : cap), dim3(BS), 0, s, tiles, reps, plan, nreps);
+    hipLaunchKernelGGL((k_build<BS, LPP>), dim3(items < cap ? items : cap), dim3(BS), 0, s, tiles, reps, plan, nreps,
+                       ntiles);
 }
 
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
                    bool bookkeep)
 {
     if (ntiles <= 0) return 0;
-    if (!g.plan || !build_shape_valid(g.build_bs, g.build_lpp)) return -1;
+    if (!build_shape_valid(g.build_bs, g.build_lpp)) return -1;
+    // one replica: no work list, every rebuild kernel checks the flag itself
+    const int *plan = nreps == 1 ? nullptr : g.plan;
+    if (nreps > 1 && !plan) return -1;
     const int gt = ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP;
-    hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, g.plan);
-    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, g.plan, nreps);
-    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, g.plan);
-    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, g.plan);
-    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, g.plan, nreps);
-    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, g.plan, nreps);
+    if (plan) hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, g.plan);
+    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
+    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan);
+    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan);
+    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
+    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
     if (g.nruns > 0) {
         const int gr = g.nruns < SC_RUN_GRID_CAP ? g.nruns : SC_RUN_GRID_CAP;
         hipLaunchKernelGGL(k_build_cells, dim3(gr), dim3(SC_TB), 0, s, reps, nreps, g.nruns);
@@ -1867,14 +1898,14 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     if (g.any_tile_build) {
         const int key = g.build_bs * 16 + g.build_lpp;
         switch (key) {
-        case 64 * 16 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        case 64 * 16 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        case 64 * 16 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        case 64 * 16 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        case 256 * 16 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        case 256 * 16 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        case 256 * 16 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, g.plan, s); break;
-        default: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, g.plan, s); break;
+        case 64 * 16 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 64 * 16 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 64 * 16 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 64 * 16 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 256 * 16 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 256 * 16 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 256 * 16 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, s); break;
+        default: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, s); break;
         }
     }
     if (bookkeep) fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);

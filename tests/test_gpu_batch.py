@@ -105,3 +105,31 @@ def test_admission_bytes_and_info():
     b.add(p, D, r, v)
     info = b.info()
     assert info["active"] == 1 and info["particles"] == p.N and info["device_bytes"] == need
+
+
+def test_stream_group_and_async_match_single_batch():
+    ref, _ = run(SPECS, 80)
+    g = nat.BatchGroup(16, streams=2, graph=True)
+    ids = []
+    for s in SPECS:
+        p, D, r, v = W.replica(**s)
+        ids.append(g.add(p, D, r, v))
+    g.minimize(*RELAX)
+    g.step(30)
+    g.step(50)
+    assert g.info()["active"] == len(SPECS)
+    for k, gid in enumerate(ids):
+        got = g.get_state(gid)
+        for f in ("r", "v", "F"):
+            assert np.array_equal(got[f], ref[k][f]), (k, f)
+    g.close()
+    # a call made while steps are in flight waits for them first
+    b = nat.Batch(4)
+    p, D, r, v = W.replica(**SPECS[0])
+    rid = b.add(p, D, r, v)
+    b.minimize(*RELAX)
+    b.step_async(80)
+    st = b.stats(rid)
+    assert st["steps"] == 80
+    assert np.array_equal(b.get_state(rid)["r"], ref[0]["r"])
+    b.close()
