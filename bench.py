@@ -41,6 +41,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from a committed counter summary
+    (profiles/<name>.txt, line 'bytes_per_launch = B': FETCH_SIZE and
+    WRITE_SIZE per dispatch, each scaled by a copy kernel's reading), or None
+    when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", name + ".txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
 METRIC = "particle-steps/s FP64 vs mixed @ stated force err; p50/p99 replica latency"
 
 
@@ -148,7 +162,8 @@ def bench_large(args, nat, synth):
     bytes_per_particle = 24 + 8 + 48 + 24 + 4 * (n_half + 1)
     achieved = N * bytes_per_particle / f_avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("traffic_k_force1_berthier3d_N1e6_fp64"),
                 "kernel": "k_force<1> (pair forces + second half-kick + COM partials)",
                 "kernel_avg_us": round(f_avg_s * 1e6, 2), "bytes_per_particle": round(bytes_per_particle, 2),
                 "n_half_mean": round(n_half, 3),
@@ -216,7 +231,8 @@ def bench_batched(args, nat):
     specs = W.batched_specs(args.replicas)
     t0 = time.perf_counter()
     if args.cell_divisor:
-        cd, shape, tune = args.cell_divisor, (256, 4), None
+        shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else (256, 4)
+        cd, tune = args.cell_divisor, None
     else:
         cd, shape, tune = W.autotune_batch(specs, pilot=64, steps=60)
     log(f"[bench] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
@@ -267,14 +283,17 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--N", type=int, default=10 ** 6)
-    ap.add_argument("--workload", choices=("large", "batched", "served"), default="large")
+    ap.add_argument("--workload", choices=("large", "batched", "served", "copycal"), default="large",
+                    help="copycal: five plain 256 MiB device copies, the reference reading for HBM counters")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=4000)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20261206)
-    ap.add_argument("--cell-divisor", type=int, default=0, help="0: library default")
+    ap.add_argument("--cell-divisor", type=int, default=0,
+                    help="0: library default (large) / autotuned (batched)")
+    ap.add_argument("--shape", default="", help="batched with --cell-divisor: build launch shape BS,LPP")
     ap.add_argument("--rate", type=float, nargs="*", default=None, help="served: arrival rates (jobs/s)")
     ap.add_argument("--jobs", type=int, default=48, help="served: jobs per arrival rate")
     args = ap.parse_args()
@@ -282,6 +301,15 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md for how the hot path would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "copycal":
+        import torch
+        a = torch.empty(1 << 26, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("[bench] copycal: 5 copies of 256 MiB")
+        return
     from simulationcode_amd import _native as nat, synth
     nat.load()
     if args.workload == "large":
