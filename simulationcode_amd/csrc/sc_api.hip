@@ -179,12 +179,13 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     h.tile0 = 0;
     h.run0 = 0;
     c->geom.max_chunks = (h.ncell + SC_CHUNK - 1) / SC_CHUNK;
-    c->geom.nruns = h.nruns;
-    c->geom.any_tile_build = h.lds_build ? 0 : 1;
+    c->geom.nruns = h.build_kind == BUILD_CELLS ? h.nruns : 0;
+    c->geom.nwin = h.build_kind == BUILD_WIN ? h.nruns : 0;
+    c->geom.any_tile_build = h.build_kind == BUILD_SLOT ? 1 : 0;
     rc = SC_OK;
     rc |= dalloc(c, &c->d_rep, 1);
     rc |= dalloc(c, &c->d_tiles, c->ntiles);
-    rc |= dalloc(c, &c->geom.plan, 2 + 1 + c->ntiles);
+    rc |= dalloc(c, &c->geom.plan, sc::plan_ints(1, c->ntiles, c->geom.nwin));
     if (rc != SC_OK) return SC_ERR_OOM;
     std::vector<Tile> tiles(c->ntiles);
     for (int t = 0; t < c->ntiles; t++) tiles[t] = Tile{0, t * SC_TB, t, 0};
@@ -216,7 +217,17 @@ int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, c
     if (r && (rc = upload_field(c, r, FLD_R))) return rc;
     if (v && (rc = upload_field(c, v, FLD_V))) return rc;
     if (F && (rc = upload_field(c, F, FLD_F))) return rc;
-    if (D && (rc = upload_field(c, D, FLD_D))) return rc;
+    if (D) {
+        if ((rc = upload_field(c, D, FLD_D))) return rc;
+        // the windowed build bounds every pair's half-list radius by the
+        // largest diameter (swap moves keep the set of diameters)
+        float dm = c->h.dmax;
+        for (int64_t i = 0; i < c->h.n; i++) dm = std::max(dm, (float)D[i]);
+        if (dm > c->h.dmax) {
+            c->h.dmax = dm;
+            if ((rc = push_rep(c))) return rc;
+        }
+    }
     if (disp && (rc = upload_field(c, disp, FLD_DISP))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->s));
     return SC_OK;

@@ -106,11 +106,12 @@ int sync_tables(sc_batch *b)
         Slot &sl = b->slots[b->act[r]];
         sl.h.tile0 = (int)tiles.size();
         sl.h.run0 = run0;
-        run0 += sl.h.nruns;
+        if (sl.h.build_kind == BUILD_CELLS) run0 += sl.h.nruns;
+        if (sl.h.build_kind == BUILD_WIN) b->geom.nwin += sl.h.nruns;
         for (int t = 0; t < sl.h.ntiles; t++) tiles.push_back(Tile{r, t * SC_TB, t, 0});
         reps.push_back(sl.h);
         b->geom.max_chunks = std::max(b->geom.max_chunks, (sl.h.ncell + SC_CHUNK - 1) / SC_CHUNK);
-        if (!sl.h.lds_build) b->geom.any_tile_build = 1;
+        if (sl.h.build_kind == BUILD_SLOT) b->geom.any_tile_build = 1;
         if (sl.h.sys == SC_SYS_BROWNIAN)
             b->has_bd = true;
         else
@@ -119,7 +120,7 @@ int sync_tables(sc_batch *b)
     }
     b->geom.nruns = run0;
     b->ntiles = (int)tiles.size();
-    const int plan_need = 2 + b->nact + b->ntiles;
+    const int plan_need = sc::plan_ints(b->nact, b->ntiles, b->geom.nwin);
     if (plan_need > b->plan_cap) {
         const int cap = std::max(plan_need, 2 * b->plan_cap);
         if (b->d_plan) (void)hipFree(b->d_plan);

@@ -24,6 +24,12 @@
 #define SC_WMAX 1024                    // particles of one LDS-staged build window
 #define SC_GRID_CAP 2048                // blocks of the grid-stride rebuild kernels
 #define SC_RUN_GRID_CAP 1024            // blocks of the LDS build
+#define SC_WIN_MAX 2048                 // particles of one windowed-build LDS window
+#define SC_WIN_CELLS 768                // cells of one window
+#define SC_WIN_ROWS 64                  // own cell rows of one window run
+
+// neighbour-list build kinds of a replica
+enum { BUILD_SLOT = 0, BUILD_CELLS = 1, BUILD_WIN = 2 };
 
 // error bits raised on the device
 #define SC_EBIT_OVERFLOW 1
@@ -58,9 +64,15 @@ struct Rep {
     int cap;             // maxneighbours (row capacity)
     int rescale;
     int ntiles, tile0;   // tiles of this replica in the launch table
-    int lds_build;       // 1: LDS-staged window build (stencil +-1); 0: per-slot build
-    int cx;              // cells per LDS build run (along x)
-    int nruns, run0;     // LDS build runs of this replica and their offset in the launch
+    int build_kind;      // BUILD_SLOT: per-slot sweep; BUILD_CELLS: LDS cell runs (stencil +-1);
+                         // BUILD_WIN: LDS windows over cx x cy x cz own cells (any stencil)
+    int cx, cy, cz;      // own cells per build run along x, y, z
+    int nruns, run0;     // build runs of this replica and their offset in the launch
+    float wlo, whi;      // windowed build: FP32 d^2 below wlo is certainly <= R_v^2, above whi
+                         // certainly > R_v^2; in between the FP64 test decides
+    float hmarg;         // same margin for the half-list radius
+    float dmax;          // largest diameter held so far (bounds every pair's half-list radius)
+    double wcell;        // cell side L / nc
     double L, invL, cinv;          // box, 1/box, cells per unit length
     double fc2, rcut, skin, Rv2, thr;
     unsigned long long thr_bits;  // bit pattern of thr (= (skin/2)^2)
@@ -102,13 +114,17 @@ int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nr
 int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 struct RebuildGeom {
     int max_chunks = 0;      // max over replicas of ceil(ncell / SC_CHUNK)
-    int nruns = 0;           // LDS build runs over all replicas
-    int any_tile_build = 0;  // some replica uses the per-slot build (all-pairs / wide stencil)
-    int *plan = nullptr;     // device work list, 2 + nreps + ntiles ints (k_rebuild_plan)
+    int nruns = 0;           // LDS cell-run build runs over all replicas (BUILD_CELLS)
+    int nwin = 0;            // windowed-build runs over all replicas (BUILD_WIN)
+    int any_tile_build = 0;  // some replica uses the per-slot build (all-pairs / explicit)
+    int *plan = nullptr;     // device work list, plan_ints(nreps, ntiles, nwin) ints (k_rebuild_plan)
     int build_bs = 256;      // per-slot build launch shape: threads per block (64 or 256)
     int build_lpp = 1;       //   and lanes per slot (1, 2, 4 or 8); the lists do not depend on it
 };
 bool build_shape_valid(int bs, int lpp);
+// rebuild work list: header {tiles, replicas, window runs, 0}, then the
+// replicas, the tiles and the (replica, run) pairs of the rebuilding replicas
+inline int plan_ints(int nreps, int ntiles, int nwin) { return 4 + nreps + ntiles + 2 * nwin; }
 // the per-slot build launch shapes tried by the autotuner, {block, lanes}
 constexpr int kBuildShapes[8][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {64, 1}, {64, 2}, {64, 4}, {64, 8}};
 // bookkeep=false when the next launch is a velocity-Verlet force finish,

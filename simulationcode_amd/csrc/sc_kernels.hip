@@ -18,6 +18,42 @@
 
 #include <math.h>
 
+// Device buffers are reached through the pointers of a Rep read from memory,
+// which the compiler cannot place in an address space: it would emit FLAT
+// instructions, which also count against the LDS wait counter (an LDS read
+// then waits for every outstanding global access).  gp() asserts the global
+// address space so every such access is a global_load / global_store.
+#define SC_AS1 __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ SC_AS1 T *gp(T *p)
+{
+    return (SC_AS1 T *)p;
+}
+
+// double4 loads / stores through global pointers (the vector type's own
+// copy operations take generic references)
+typedef double sc_dbl4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double4 ld4(const SC_AS1 double4 *p)
+{
+    const sc_dbl4v v = *(const SC_AS1 sc_dbl4v *)p;
+    return make_double4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void st4(SC_AS1 double4 *p, const double4 &x)
+{
+    *(SC_AS1 sc_dbl4v *)p = sc_dbl4v{x.x, x.y, x.z, x.w};
+}
+
+// Relaxed device-scope atomics on global memory (the semantics of HIP's
+// atomicAdd / atomicOr).
+__device__ __forceinline__ int gatomic_add(SC_AS1 int *p, int v)
+{
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void gatomic_or(SC_AS1 int *p, int v)
+{
+    (void)__hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 namespace {
 
 constexpr int WAVE = 64;
@@ -108,11 +144,11 @@ __device__ inline int xcd_tile()
 // decision is needed, so a tile whose own maximum exceeds the threshold sets
 // the flag (max(...) > thr  <=>  some tile's max > thr); ordinary steps issue
 // no atomics at all.
-__device__ inline bool need_rebuild(const Rep &R) { return R.dyn->rebuild != 0; }
+__device__ inline bool need_rebuild(const Rep &R) { return gp(R.dyn)->rebuild != 0; }
 
 __device__ inline void publish_maxdisp(const Rep &R, double m)
 {
-    if (threadIdx.x == 0 && m > R.thr) atomicOr(&R.dyn->rebuild, 1);
+    if (threadIdx.x == 0 && m > R.thr) gatomic_or(&gp(R.dyn)->rebuild, 1);
 }
 
 // Per-tile partial slots (8 doubles per tile in Rep::part).
@@ -120,7 +156,7 @@ enum { PS_VEC = 0, PS_MAXD2 = 3, PS_AUXMAX = 4, PS_ESUM = 5, PS_ECOMP = 6, PS_SC
 
 __device__ inline void put_partial(const Rep &R, const Tile &t, int slot, double x)
 {
-    if (threadIdx.x == 0) R.part[slot * R.ntiles + t.local] = x;
+    if (threadIdx.x == 0) gp(R.part)[slot * R.ntiles + t.local] = x;
 }
 
 // ------------------------------------------------------------ Philox4x32-10
@@ -367,35 +403,35 @@ __device__ inline int wrapc(int c, int nc) { return c < 0 ? c + nc : (c >= nc ? 
 template <int DIMS>
 __device__ void kick_drift_tile(const Rep &R, const Tile &t)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n;
     const int i = t.first + threadIdx.x;
     const int pend = dyn->pending;
     double s2 = 0.0;
     if (i < n) {
-        double4 p = R.pos[i];
+        double4 p = ld4(gp(R.pos) + (i));
         double pc[3] = {p.x, p.y, p.z};
         double s = 0.0;
         bool fin = true;
 #pragma unroll
         for (int c = 0; c < DIMS; c++) {
-            double vc = R.v[c * n + i];
+            double vc = gp(R.v)[c * n + i];
             if (pend & 1) vc = vc - dyn->vmean[c];
             if (pend & 2) vc = vc * dyn->vscale;
-            vc = vc + R.dt * R.F[c * n + i] / (2.0 * R.m);
+            vc = vc + R.dt * gp(R.F)[c * n + i] / (2.0 * R.m);
             double dr = R.dt * vc;
-            double dc = R.disp[c * n + i] + dr;
+            double dc = gp(R.disp)[c * n + i] + dr;
             s = (c == 0) ? dc * dc : s + dc * dc;
             double rn = pc[c] + dr;
             rn -= floor(rn / R.L) * R.L;
             pc[c] = rn;
             fin = fin && isfinite(rn);
-            R.v[c * n + i] = vc;
-            R.disp[c * n + i] = dc;
+            gp(R.v)[c * n + i] = vc;
+            gp(R.disp)[c * n + i] = dc;
         }
-        R.pos[i] = make_double4(pc[0], pc[1], pc[2], p.w);
+        st4(gp(R.pos) + (i), make_double4(pc[0], pc[1], pc[2], p.w));
         s2 = s;
-        if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
+        if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }
     publish_maxdisp(R, block_max(s2));
 }
@@ -416,32 +452,32 @@ __global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const R
 template <int DIMS>
 __device__ void drift_brownian_tile(const Rep &R, const Tile &t)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n;
     const int i = t.first + threadIdx.x;
     double s2 = 0.0;
     if (i < n) {
-        double4 p = R.pos[i];
+        double4 p = ld4(gp(R.pos) + (i));
         double pc[3] = {p.x, p.y, p.z};
         double factor = R.dt / R.gamma;
         double s = 0.0;
         bool fin = true;
 #pragma unroll
         for (int c = 0; c < DIMS; c++) {
-            double fc = R.F[c * n + i] - dyn->Fmean[c];
+            double fc = gp(R.F)[c * n + i] - dyn->Fmean[c];
             double dr = factor * fc;
-            double dc = R.disp[c * n + i] + dr;
+            double dc = gp(R.disp)[c * n + i] + dr;
             s = (c == 0) ? dc * dc : s + dc * dc;
             double rn = pc[c] + dr;
             rn -= floor(rn / R.L) * R.L;
             pc[c] = rn;
             fin = fin && isfinite(rn);
-            R.F[c * n + i] = fc;
-            R.disp[c * n + i] = dc;
+            gp(R.F)[c * n + i] = fc;
+            gp(R.disp)[c * n + i] = dc;
         }
-        R.pos[i] = make_double4(pc[0], pc[1], pc[2], p.w);
+        st4(gp(R.pos) + (i), make_double4(pc[0], pc[1], pc[2], p.w));
         s2 = s;
-        if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
+        if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }
     publish_maxdisp(R, block_max(s2));
 }
@@ -466,16 +502,16 @@ __global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, con
 template <int DIMS, int POT, int PREC, int MODE>
 __device__ void force_tile(const Rep &R, const Tile &t)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n;
     const int i = t.first + threadIdx.x;
     const long long step = dyn->step;
     double f[3] = {0.0, 0.0, 0.0};
     double red[3] = {0.0, 0.0, 0.0};
     if (i < n) {
-        const double4 pi = R.pos[i];
-        const int cnt = R.nn[i];
-        const int *nb = R.nbr + i;
+        const double4 pi = ld4(gp(R.pos) + (i));
+        const int cnt = gp(R.nn)[i];
+        const SC_AS1 int *nb = gp(R.nbr) + i;
         if (PREC == 0) {
             // gathers issued four at a time so several are in flight per
             // lane; accumulation order stays k = 0, 1, 2, ...
@@ -486,7 +522,7 @@ __device__ void force_tile(const Rep &R, const Tile &t)
 #pragma unroll
                 for (int u = 0; u < 4; u++) jj[u] = nb[(size_t)(k + u) * n];
 #pragma unroll
-                for (int u = 0; u < 4; u++) pp[u] = R.pos[jj[u]];
+                for (int u = 0; u < 4; u++) pp[u] = ld4(gp(R.pos) + (jj[u]));
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     double dx, dy, dz;
@@ -499,7 +535,7 @@ __device__ void force_tile(const Rep &R, const Tile &t)
             }
             for (; k < cnt; k++) {
                 const int j = nb[(size_t)k * n];
-                const double4 pj = R.pos[j];
+                const double4 pj = ld4(gp(R.pos) + (j));
                 double dx, dy, dz;
                 double rij2 = sqdist<DIMS>(R, pi, pj, dx, dy, dz);
                 double fs = pair_fdivr64<DIMS, POT>(R, rij2, pi.w, pj.w);
@@ -518,7 +554,7 @@ __device__ void force_tile(const Rep &R, const Tile &t)
 #pragma unroll
                 for (int u = 0; u < 4; u++) jj[u] = nb[(size_t)(k + u) * n];
 #pragma unroll
-                for (int u = 0; u < 4; u++) pp[u] = R.pos[jj[u]];
+                for (int u = 0; u < 4; u++) pp[u] = ld4(gp(R.pos) + (jj[u]));
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const double dx = min_image(pi.x - pp[u].x, R.L, R.invL);
@@ -535,7 +571,7 @@ __device__ void force_tile(const Rep &R, const Tile &t)
             }
             for (; k < cnt; k++) {
                 const int j = nb[(size_t)k * n];
-                const double4 pj = R.pos[j];
+                const double4 pj = ld4(gp(R.pos) + (j));
                 const double dx = min_image(pi.x - pj.x, R.L, R.invL);
                 const double dy = min_image(pi.y - pj.y, R.L, R.invL);
                 const double dz = DIMS == 3 ? min_image(pi.z - pj.z, R.L, R.invL) : 0.0;
@@ -552,34 +588,34 @@ __device__ void force_tile(const Rep &R, const Tile &t)
         if (R.sys != 0) {
             double xi[3];
             if (R.noise) {
-                const double *nz = R.noise + (size_t)(step - R.noise_step0) * (size_t)n * DIMS +
-                                   (size_t)R.orig[i] * DIMS;
+                const SC_AS1 double *nz = gp(R.noise) + (size_t)(step - R.noise_step0) * (size_t)n * DIMS +
+                                   (size_t)gp(R.orig)[i] * DIMS;
 #pragma unroll
                 for (int c = 0; c < DIMS; c++) xi[c] = nz[c];
             } else {
                 double z0, z1, z2, z3;
-                normals2((uint32_t)R.orig[i], step, 0u, R.seed, z0, z1);
+                normals2((uint32_t)gp(R.orig)[i], step, 0u, R.seed, z0, z1);
                 xi[0] = z0;
                 xi[1] = z1;
                 if (DIMS == 3) {
-                    normals2((uint32_t)R.orig[i], step, 1u, R.seed, z2, z3);
+                    normals2((uint32_t)gp(R.orig)[i], step, 1u, R.seed, z2, z3);
                     xi[2] = z2;
                 }
             }
             double factor = sqrt(2.0 * R.kBT * R.gamma / R.dt);
 #pragma unroll
             for (int c = 0; c < DIMS; c++) {
-                double add = (R.sys == 1) ? -R.gamma * R.v[c * n + i] + factor * xi[c] : factor * xi[c];
+                double add = (R.sys == 1) ? -R.gamma * gp(R.v)[c * n + i] + factor * xi[c] : factor * xi[c];
                 f[c] = f[c] + add;
             }
         }
 #pragma unroll
-        for (int c = 0; c < DIMS; c++) R.F[c * n + i] = f[c];
+        for (int c = 0; c < DIMS; c++) gp(R.F)[c * n + i] = f[c];
         if (MODE == 1) {
 #pragma unroll
             for (int c = 0; c < DIMS; c++) {
-                double vc = R.v[c * n + i] + R.dt * f[c] / (2.0 * R.m);
-                R.v[c * n + i] = vc;
+                double vc = gp(R.v)[c * n + i] + R.dt * f[c] / (2.0 * R.m);
+                gp(R.v)[c * n + i] = vc;
                 red[c] = vc;
             }
         } else if (MODE == 2) {
@@ -591,7 +627,7 @@ __device__ void force_tile(const Rep &R, const Tile &t)
         block_sum<3>(red);
         if (threadIdx.x == 0)
 #pragma unroll
-            for (int c = 0; c < 3; c++) R.part[(PS_VEC + c) * R.ntiles + t.local] = red[c];
+            for (int c = 0; c < 3; c++) gp(R.part)[(PS_VEC + c) * R.ntiles + t.local] = red[c];
     }
 }
 
@@ -619,14 +655,14 @@ __global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *r
 template <int DIMS>
 __device__ void ekin_com_tile(const Rep &R, const Tile &t)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n, i = t.first + threadIdx.x;
     double e[1] = {0.0};
     if (i < n) {
         double s = 0.0;
 #pragma unroll
         for (int c = 0; c < DIMS; c++) {
-            double vc = R.v[c * n + i] - dyn->vmean[c];
+            double vc = gp(R.v)[c * n + i] - dyn->vmean[c];
             s = (c == 0) ? vc * vc : s + vc * vc;
         }
         e[0] = s;
@@ -652,15 +688,15 @@ __global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int pend = dyn->pending;
     const int n = R.n, i = t.first + threadIdx.x;
     if (pend && i < n) {
         for (int c = 0; c < R.dims; c++) {
-            double vc = R.v[c * n + i];
+            double vc = gp(R.v)[c * n + i];
             if (pend & 1) vc = vc - dyn->vmean[c];
             if (pend & 2) vc = vc * dyn->vscale;
-            R.v[c * n + i] = vc;
+            gp(R.v)[c * n + i] = vc;
         }
     }
 }
@@ -674,47 +710,52 @@ __global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep
 __global__ void k_request_rebuild(Rep *reps, int nreps)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nreps) reps[r].dyn->rebuild = 1;
+    if (r < nreps) gp(reps[r].dyn)->rebuild = 1;
 }
 
 // Error bits of every replica into one array (one host copy per check).
 __global__ void k_gather_err(const Rep *reps, int nreps, int *out)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nreps) out[r] = reps[r].dyn->err;
+    if (r < nreps) out[r] = gp(reps[r].dyn)->err;
 }
 
 // Rebuild work list.  plan[0] = number of tiles to rebuild, plan[1] = number
-// of replicas to rebuild, plan[2 .. 2+nreps) = those replicas, ascending,
-// plan[2+nreps ..) = their tiles, ascending.  The rebuild kernels sweep this
-// list only, so a step in which a few replicas of a large batch rebuild
-// spreads just their tiles over the whole grid.
-__global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nreps, int *plan)
+// of replicas to rebuild, plan[2] = number of windowed-build runs, then from
+// plan[4]: those replicas (ascending), their tiles (ascending) and the
+// (replica, run) pairs of their windowed builds.  The rebuild kernels sweep
+// these lists only, so a step in which a few replicas of a large batch
+// rebuild spreads just their work over the whole grid.
+__global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nreps, int ntiles, int *plan)
 {
-    __shared__ int st[SC_TB], sr[SC_TB], s0[SC_TB];
-    __shared__ int base_t, base_r;
+    __shared__ int st[SC_TB], sr[SC_TB], s0[SC_TB], sw[SC_TB];
+    __shared__ int base_t, base_r, base_w;
     const int tid = threadIdx.x;
-    int *rlist = plan + 2, *tlist = plan + 2 + nreps;
-    if (tid == 0) base_t = base_r = 0;
+    int *rlist = plan + 4, *tlist = plan + 4 + nreps, *wlist = plan + 4 + nreps + ntiles;
+    if (tid == 0) base_t = base_r = base_w = 0;
     for (int b = 0; b < nreps; b += SC_TB) {
         const int r = b + tid;
-        int flag = 0, nt = 0, t0 = 0;
+        int flag = 0, nt = 0, t0 = 0, nw = 0;
         if (r < nreps) {
             const Rep &R = reps[r];
             t0 = R.tile0;
             nt = R.ntiles;
-            flag = R.dyn->rebuild != 0 ? 1 : 0;
+            flag = gp(R.dyn)->rebuild != 0 ? 1 : 0;
             nt = flag ? nt : 0;
+            nw = (flag && R.build_kind == BUILD_WIN) ? R.nruns : 0;
         }
         st[tid] = nt;
         sr[tid] = flag;
         s0[tid] = t0;
+        sw[tid] = nw;
         __syncthreads();
-        for (int o = 1; o < SC_TB; o <<= 1) {  // inclusive scans of tile and replica counts
+        for (int o = 1; o < SC_TB; o <<= 1) {  // inclusive scans of tile, replica and run counts
             const int xt = tid >= o ? st[tid - o] : 0, xr = tid >= o ? sr[tid - o] : 0;
+            const int xw = tid >= o ? sw[tid - o] : 0;
             __syncthreads();
             st[tid] += xt;
             sr[tid] += xr;
+            sw[tid] += xw;
             __syncthreads();
         }
         if (flag) rlist[base_r + sr[tid] - 1] = r;
@@ -731,16 +772,33 @@ __global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nre
             const int before = lo ? st[lo - 1] : 0;
             tlist[base_t + x] = s0[lo] + (x - before);
         }
+        const int totw = sw[SC_TB - 1];
+        for (int x = tid; x < totw; x += SC_TB) {  // x-th planned run of this chunk
+            int lo = 0, hi = SC_TB - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (sw[mid] > x)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            const int before = lo ? sw[lo - 1] : 0;
+            wlist[2 * (base_w + x)] = b + lo;
+            wlist[2 * (base_w + x) + 1] = x - before;
+        }
         __syncthreads();
         if (tid == 0) {
             base_t += total;
             base_r += sr[SC_TB - 1];
+            base_w += totw;
         }
         __syncthreads();
     }
     if (tid == 0) {
         plan[0] = base_t;
         plan[1] = base_r;
+        plan[2] = base_w;
+        plan[3] = 0;
     }
 }
 
@@ -754,22 +812,22 @@ struct RebuildWork {
 
 __device__ inline RebuildWork rebuild_work(const int *plan, int nreps, const Rep *reps, int ntiles)
 {
-    if (plan) return RebuildWork{plan + 2 + nreps, plan[0]};
-    return RebuildWork{nullptr, reps[0].dyn->rebuild != 0 ? ntiles : 0};
+    if (plan) return RebuildWork{plan + 4 + nreps, plan[0]};
+    return RebuildWork{nullptr, gp(reps[0].dyn)->rebuild != 0 ? ntiles : 0};
 }
 
 // Replica of the idx-th rebuilding replica, or -1.
 __device__ inline int rebuild_rep(const int *plan, const Rep *reps, int idx)
 {
-    if (plan) return idx < plan[1] ? plan[2 + idx] : -1;
-    return (idx == 0 && reps[0].dyn->rebuild != 0) ? 0 : -1;
+    if (plan) return idx < plan[1] ? plan[4 + idx] : -1;
+    return (idx == 0 && gp(reps[0].dyn)->rebuild != 0) ? 0 : -1;
 }
 
 // Slot range start of cell c: chunk-local exclusive prefix + chunk offset
 // (two-level scan), c == ncell gives n.
 __device__ inline int cstart(const Rep &R, int c)
 {
-    return c >= R.ncell ? R.n : R.cell_start[c] + R.chunk_off[c >> SC_CHUNK_SHIFT];
+    return c >= R.ncell ? R.n : gp(R.cell_start)[c] + gp(R.chunk_off)[c >> SC_CHUNK_SHIFT];
 }
 
 // XCD-aware grid-stride iteration for capped grids: the blocks resident on
@@ -804,14 +862,14 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
         const bool valid = i < R.n;
         int c = -1;
         if (valid) {
-            double4 p = R.pos[i];
+            double4 p = ld4(gp(R.pos) + (i));
             int cc[3];
             if (R.dims == 3)
                 cell_of_pos<3>(R, p, cc);
             else
                 cell_of_pos<2>(R, p, cc);
             c = cc[0] + R.nc * (cc[1] + R.nc * cc[2]);
-            R.cell_of[i] = c;
+            gp(R.cell_of)[i] = c;
         }
         // group the wavefront's lanes by cell (no memory traffic), then every
         // group leader reserves its group's ranks with one atomic, all
@@ -831,9 +889,9 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
             active &= ~same;
         }
         int base = 0;
-        if (group) base = atomicAdd(&R.cell_cnt[c], group);
+        if (group) base = gatomic_add(&gp(R.cell_cnt)[c], group);
         base = __shfl(base, leader_of, WAVE);
-        if (valid) R.rank[i] = base + local;
+        if (valid) gp(R.rank)[i] = base + local;
     }
 }
 
@@ -852,7 +910,7 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int c = c0 + tid * 4 + q;
-        v[q] = c < R.ncell ? R.cell_cnt[c] : 0;
+        v[q] = c < R.ncell ? gp(R.cell_cnt)[c] : 0;
         s += v[q];
     }
     __shared__ int sums[SC_TB];
@@ -869,12 +927,12 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
     for (int q = 0; q < 4; q++) {
         const int c = c0 + tid * 4 + q;
         if (c < R.ncell) {
-            R.cell_start[c] = run;
-            R.cell_cnt[c] = 0;
+            gp(R.cell_start)[c] = run;
+            gp(R.cell_cnt)[c] = 0;
         }
         run += v[q];
     }
-    if (tid == SC_TB - 1) R.chunk_off[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
+    if (tid == SC_TB - 1) gp(R.chunk_off)[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
 }
 
 // R2b: exclusive scan of the chunk totals (one block per replica).
@@ -890,7 +948,7 @@ __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *
     __syncthreads();
     for (int b = 0; b < nch; b += SC_TB) {
         const int q = b + threadIdx.x;
-        const int x = q < nch ? R.chunk_off[q] : 0;
+        const int x = q < nch ? gp(R.chunk_off)[q] : 0;
         sums[threadIdx.x] = x;
         __syncthreads();
         for (int o = 1; o < SC_TB; o <<= 1) {
@@ -899,7 +957,7 @@ __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *
             sums[threadIdx.x] += y;
             __syncthreads();
         }
-        if (q < nch) R.chunk_off[q] = carry + sums[threadIdx.x] - x;
+        if (q < nch) gp(R.chunk_off)[q] = carry + sums[threadIdx.x] - x;
         __syncthreads();
         if (threadIdx.x == 0) carry += sums[SC_TB - 1];
         __syncthreads();
@@ -917,7 +975,7 @@ __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep 
         const Rep R = reps[t.rep];
         const int i = t.first + threadIdx.x;
         if (i >= R.n) continue;
-        R.key[cstart(R, R.cell_of[i]) + R.rank[i]] = R.orig[i];
+        gp(R.key)[cstart(R, gp(R.cell_of)[i]) + gp(R.rank)[i]] = gp(R.orig)[i];
     }
 }
 
@@ -935,15 +993,15 @@ __global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const
         const Rep R = reps[t.rep];
         const int x = t.first + threadIdx.x, n = R.n;
         if (x >= n) continue;
-        const int c = R.cell_of[x];
+        const int c = gp(R.cell_of)[x];
         const int s0 = cstart(R, c), e0 = cstart(R, c + 1);
-        const int oid = R.orig[x];
+        const int oid = gp(R.orig)[x];
         int rank = 0;
-        for (int a = s0; a < e0; a++) rank += R.key[a] < oid ? 1 : 0;
+        for (int a = s0; a < e0; a++) rank += gp(R.key)[a] < oid ? 1 : 0;
         const int k = s0 + rank;
-        R.pos_alt[k] = R.pos[x];
-        for (int d = 0; d < R.dims; d++) R.v_alt[d * n + k] = R.v[d * n + x];
-        R.orig_alt[k] = oid;
+        st4(gp(R.pos_alt) + (k), ld4(gp(R.pos) + (x)));
+        for (int d = 0; d < R.dims; d++) gp(R.v_alt)[d * n + k] = gp(R.v)[d * n + x];
+        gp(R.orig_alt)[k] = oid;
     }
 }
 
@@ -968,7 +1026,7 @@ __device__ inline void build_one(const Rep &R, int k, const double4 p, int &nful
         double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
         double rh = R.rcut * md + R.skin;
         if (d2 < rh * rh) {
-            if (cnt < R.cap) R.nbr[(size_t)cnt * R.n + k] = j;
+            if (cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = j;
             cnt++;
         }
     }
@@ -978,14 +1036,14 @@ template <int DIMS>
 __device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, int nfull, int cnt)
 {
     const int n = R.n;
-    if (nfull > R.cap) atomicOr(&R.dyn->err, SC_EBIT_OVERFLOW);
-    R.nn[k] = cnt < R.cap ? cnt : R.cap;
-    R.pos[k] = p;
+    if (nfull > R.cap) gatomic_or(&gp(R.dyn)->err, SC_EBIT_OVERFLOW);
+    gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
+    st4(gp(R.pos) + (k), p);
     for (int c = 0; c < DIMS; c++) {
-        R.v[c * n + k] = R.v_alt[c * n + k];
-        R.disp[c * n + k] = 0.0;
+        gp(R.v)[c * n + k] = gp(R.v_alt)[c * n + k];
+        gp(R.disp)[c * n + k] = 0.0;
     }
-    R.orig[k] = R.orig_alt[k];
+    gp(R.orig)[k] = gp(R.orig_alt)[k];
 }
 
 // LPP lanes cooperate on one particle: each takes every LPP-th candidate of
@@ -1045,7 +1103,7 @@ __device__ void build_run(const Rep &R, int run, BuildLds &S)
                     hi = mid - 1;
             }
             const int j = S.cglob[lo] + (a - S.coff[lo]);
-            const double4 q = R.pos_alt[j];
+            const double4 q = ld4(gp(R.pos_alt) + (j));
             S.x[a] = q.x;
             S.y[a] = q.y;
             if (DIMS == 3) S.z[a] = q.z;
@@ -1065,7 +1123,7 @@ __device__ void build_run(const Rep &R, int run, BuildLds &S)
     for (int base = own0; base < own1; base += SC_TB / LPP) {
         const int k = base + grp;
         if (k >= own1) continue;
-        const double4 p = R.pos_alt[k];
+        const double4 p = ld4(gp(R.pos_alt) + (k));
         int cc[3];
         cell_of_pos<DIMS>(R, p, cc);
         const int xi = cc[0] - x0 + 1;
@@ -1086,7 +1144,7 @@ __device__ void build_run(const Rep &R, int run, BuildLds &S)
                     } else {
                         const int cq = a < S.coff[q0 + 1] ? q0 : (a < S.coff[q0 + 2] ? q0 + 1 : q0 + 2);
                         j = S.cglob[cq] + (a - S.coff[cq]);
-                        const double4 q = R.pos_alt[j];
+                        const double4 q = ld4(gp(R.pos_alt) + (j));
                         qx = q.x;
                         qy = q.y;
                         qz = q.z;
@@ -1105,7 +1163,7 @@ __device__ void build_run(const Rep &R, int run, BuildLds &S)
                             qd = S.d[a];
                             j = S.j[a];
                         } else {
-                            qd = R.pos_alt[j].w;
+                            qd = ld4(gp(R.pos_alt) + (j)).w;
                         }
                         const double md = (R.pot == 2) ? (p.w + qd) * 0.5 * (1.0 - R.zeta * fabs(p.w - qd)) : R.lj_md;
                         const double rh = R.rcut * md + R.skin;
@@ -1116,7 +1174,7 @@ __device__ void build_run(const Rep &R, int run, BuildLds &S)
                 const unsigned long long mH = __ballot(inH) & gmask;
                 if (inH) {
                     const int slot = cnt + __popcll(mH & below);
-                    if (slot < R.cap) R.nbr[(size_t)slot * R.n + k] = j;
+                    if (slot < R.cap) gp(R.nbr)[(size_t)slot * R.n + k] = j;
                 }
                 nfull += __popcll(mF);
                 cnt += __popcll(mH);
@@ -1133,7 +1191,7 @@ __global__ __launch_bounds__(SC_TB) void k_build_cells(const Rep *reps, int nrep
         int r = 0;
         while (r + 1 < nreps && reps[r + 1].run0 <= it.cur) r++;
         const Rep R = reps[r];
-        if (!R.lds_build || !need_rebuild(R)) continue;
+        if (R.build_kind != BUILD_CELLS || !need_rebuild(R)) continue;
         if (R.dims == 3)
             build_run<3, 8>(R, it.cur - R.run0, S);
         else
@@ -1165,7 +1223,7 @@ __device__ inline void build_one_img(const Rep &R, int k, const double4 p, int &
         const double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
         const double rh = R.rcut * md + R.skin;
         if (d2 < rh * rh) {
-            if (cnt < R.cap) R.nbr[(size_t)cnt * R.n + k] = j;
+            if (cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = j;
             cnt++;
         }
     }
@@ -1194,14 +1252,14 @@ __device__ inline void build_range(const Rep &R, int k, const double4 p, int &nf
 {
     int j = b;
     for (; j + 4 <= e; j += 4) {
-        const double4 q0 = R.pos_alt[j], q1 = R.pos_alt[j + 1], q2 = R.pos_alt[j + 2], q3 = R.pos_alt[j + 3];
+        const double4 q0 = ld4(gp(R.pos_alt) + (j)), q1 = ld4(gp(R.pos_alt) + (j + 1)), q2 = ld4(gp(R.pos_alt) + (j + 2)), q3 = ld4(gp(R.pos_alt) + (j + 3));
         if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, q0, sx, sy, sz);
         if (j + 1 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 1, q1, sx, sy, sz);
         if (j + 2 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 2, q2, sx, sy, sz);
         if (j + 3 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 3, q3, sx, sy, sz);
     }
     for (; j < e; j++)
-        if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j], sx, sy, sz);
+        if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, ld4(gp(R.pos_alt) + (j)), sx, sy, sz);
 }
 
 // The stencil of slot k's cell as (2s+1)^(dims-1) rows of 2s+1 cells along
@@ -1241,11 +1299,11 @@ __device__ inline void for_stencil_rows(const Rep &R, const double4 p, Seg &&seg
 template <int DIMS>
 __device__ void build_slot(const Rep &R, int k)
 {
-    const double4 p = R.pos_alt[k];
+    const double4 p = ld4(gp(R.pos_alt) + (k));
     int nfull = 0, cnt = 0;
     if (R.srange == 0) {  // single cell: every other slot, reference minimum image
         for (int j = 0; j < R.n; j++)
-            if (j != k) build_one<DIMS>(R, k, p, nfull, cnt, j, R.pos_alt[j]);
+            if (j != k) build_one<DIMS>(R, k, p, nfull, cnt, j, ld4(gp(R.pos_alt) + (j)));
     } else {
         for_stencil_rows<DIMS>(R, p, [&](int b, int e, double sx, double sy, double sz) {
             build_range<DIMS>(R, k, p, nfull, cnt, b, e, sx, sy, sz);
@@ -1268,7 +1326,7 @@ __device__ inline void build_segment(const Rep &R, int k, const double4 p, int b
         const int a = a0 + l;
         bool inF = false, inH = false;
         if (a < e && a != k) {
-            const double4 q = R.pos_alt[a];
+            const double4 q = ld4(gp(R.pos_alt) + (a));
             double d2;
             if (IMG) {
                 const double dx = (p.x - q.x) - sx;
@@ -1293,7 +1351,7 @@ __device__ inline void build_segment(const Rep &R, int k, const double4 p, int b
         const unsigned long long mH = __ballot(inH) & gmask;
         if (inH) {
             const int slot = cnt + __popcll(mH & below);
-            if (slot < R.cap) R.nbr[(size_t)slot * R.n + k] = a;
+            if (slot < R.cap) gp(R.nbr)[(size_t)slot * R.n + k] = a;
         }
         nfull += __popcll(mF);
         cnt += __popcll(mH);
@@ -1303,7 +1361,7 @@ __device__ inline void build_segment(const Rep &R, int k, const double4 p, int b
 template <int DIMS, int LPP>
 __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long gmask, unsigned long long below)
 {
-    const double4 p = R.pos_alt[k];
+    const double4 p = ld4(gp(R.pos_alt) + (k));
     int nfull = 0, cnt = 0;
     if (R.srange == 0) {
         build_segment<DIMS, LPP, false>(R, k, p, 0, R.n, 0.0, 0.0, 0.0, l, gmask, below, nfull, cnt);
@@ -1332,7 +1390,7 @@ __global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps
     for (XcdRange it = xcd_range(wk.count * IPT); it.cur < it.end; it.cur += it.step) {
         const Tile t = tiles[wk.tile(it.cur / IPT)];
         const Rep R = reps[t.rep];
-        if (R.lds_build) continue;
+        if (R.build_kind != BUILD_SLOT) continue;
         const int k = t.first + (it.cur % IPT) * SPI + (int)threadIdx.x / LPP;
         if (k >= R.n) continue;
         if (LPP == 1) {
@@ -1349,6 +1407,411 @@ __global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps
     }
 }
 
+// R5 (windowed form, any stencil +-s, s <= 3).  A run is cx x cy x cz own
+// cells (cz = 1 in 2D); its window -- the own cells and s more cells on
+// every side, wrapped -- is staged in LDS once: window-relative FP32
+// coordinates with the cell's image shift applied, FP32 D and the slot index.
+// Each own particle is swept by WIN_LW lanes: they cut every stencil row
+// (rows oz, oy in the per-slot sweep's order) to the run of cells that lie
+// within the particle's largest half-list radius, lay the surviving rows end
+// to end, and walk that flat candidate list WIN_LW entries at a time.  The
+// list order is the per-slot sweep's (rows in order, cells along x in offset
+// order, slots in order), minus candidates no predicate could accept, so
+// the lists are identical to that sweep's.  The FP32 d^2 decides every
+// candidate outside a thin band around R_v^2 and around the half-list radius
+// (margins from the coordinate magnitudes, Rep::wlo/whi/hmarg); inside a band
+// the reference's FP64 d^2 decides, from global memory.  A window holding
+// more than SC_WIN_MAX particles sweeps its own particles from global memory.
+constexpr int WIN_LW = 8;                 // lanes per own particle
+constexpr int WIN_GROUPS = SC_TB / WIN_LW;
+constexpr int WIN_RMAX = 49;              // stencil rows per particle, (2s+1)^2 with s <= 3
+constexpr int WIN_RPL = (WIN_RMAX + WIN_LW - 1) / WIN_LW;  // rows per lane
+constexpr int WIN_SB = 2;                 // staging loads in flight per thread
+constexpr int WIN_ABITS = 11;             // LDS index bits of a row entry (SC_WIN_MAX = 2^11)
+static_assert(SC_WIN_MAX == 1 << WIN_ABITS, "row entry packing");
+
+struct WinLds {
+    float4 q[SC_WIN_MAX];
+    int j[SC_WIN_MAX];
+    int coff[SC_WIN_CELLS + 1];
+    int cglob[SC_WIN_CELLS];
+    int orow[SC_WIN_ROWS + 1];
+    int wsum[NWAVES];
+    int rows[WIN_GROUPS][WIN_RMAX + 1];  // per lane group: first LDS index | flat offset << WIN_ABITS
+};
+
+// Cell id and image shift of window cell w.
+template <int DIMS>
+__device__ inline int win_cell(const Rep &R, int w, int WX, int WY, int X0, int Y0, int Z0, double &sx, double &sy,
+                               double &sz)
+{
+    const int s = R.srange, nc = R.nc;
+    const int wx = w % WX, wy = (w / WX) % WY, wz = w / (WX * WY);
+    const int cx = wrap_shift(X0 - s + wx, nc, R.L, sx);
+    const int cy = wrap_shift(Y0 - s + wy, nc, R.L, sy);
+    int cz = 0;
+    sz = 0.0;
+    if (DIMS == 3) cz = wrap_shift(Z0 - s + wz, nc, R.L, sz);
+    return cx + nc * (cy + nc * cz);
+}
+
+// Lower bound on the distance along one axis from window-relative coordinate
+// x to window cell c (which spans [c w, (c+1) w)), less a slack eps.
+__device__ inline float cell_gap(int c, float x, float w, float eps)
+{
+    const float g = fmaxf(fmaxf((float)c * w - x, x - (float)(c + 1) * w), 0.0f) - eps;
+    return fmaxf(g, 0.0f);
+}
+
+// The reference predicates in FP64 (add_pair! and its caller's cutoff).
+template <int DIMS>
+__device__ inline void win_exact(const Rep &R, const double4 p, int j, bool &inF, bool &inH)
+{
+    const double4 q = ld4(gp(R.pos_alt) + (j));
+    double dx, dy, dz;
+    const double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
+    inF = d2 <= R.Rv2;
+    inH = false;
+    if (inF) {
+        const double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
+        const double rh = R.rcut * md + R.skin;
+        inH = d2 < rh * rh;
+    }
+}
+
+// Window x-cell range [lo, hi] of stencil row (dy, dz) within radius^2 R2 of
+// the particle (pf, window cell x wxk); lo > hi when the row lies beyond it.
+struct WinGeo {
+    float wf, winv, eps;
+    int s;
+};
+
+__device__ inline void win_row_range(const WinGeo &G, float4 pf, int wxk, float g2, float R2, int &lo, int &hi)
+{
+    lo = 1;
+    hi = 0;
+    if (g2 > R2) return;
+    const float rx = sqrtf(R2 - g2) + G.eps;
+    lo = max(wxk - G.s, (int)floorf((pf.x - rx) * G.winv));
+    hi = min(wxk + G.s, (int)floorf((pf.x + rx) * G.winv));
+}
+
+template <int DIMS>
+__device__ void win_run(const Rep &R, int run, WinLds &S)
+{
+    static_assert(SC_WIN_CELLS <= 4 * SC_TB, "four window cells per thread");
+    constexpr int LW = WIN_LW;
+    const int nc = R.nc, s = R.srange, tid = threadIdx.x;
+    const int nrx = (nc + R.cx - 1) / R.cx, nry = (nc + R.cy - 1) / R.cy;
+    const int rx = run % nrx, ry = (run / nrx) % nry, rz = run / (nrx * nry);
+    const int X0 = rx * R.cx, Y0 = ry * R.cy, Z0 = DIMS == 3 ? rz * R.cz : 0;
+    const int cxr = min(R.cx, nc - X0), cyr = min(R.cy, nc - Y0), czr = DIMS == 3 ? min(R.cz, nc - Z0) : 1;
+    const int WX = cxr + 2 * s, WY = cyr + 2 * s, WZ = DIMS == 3 ? czr + 2 * s : 1;
+    const int ncw = WX * WY * WZ;
+    __syncthreads();  // the previous run's readers are done with the LDS
+    // window cell populations and their exclusive scan, four cells per thread
+    int cnt[4], tsum = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int w = tid * 4 + q;
+        cnt[q] = 0;
+        if (w < ncw) {
+            double sx, sy, sz;
+            const int c = win_cell<DIMS>(R, w, WX, WY, X0, Y0, Z0, sx, sy, sz);
+            const int g0 = cstart(R, c);
+            S.cglob[w] = g0;
+            cnt[q] = cstart(R, c + 1) - g0;
+        }
+        tsum += cnt[q];
+    }
+    const int lane = tid % WAVE, wv = tid / WAVE;
+    int incl = tsum;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(incl, o, WAVE);
+        if (lane >= o) incl += y;
+    }
+    if (lane == WAVE - 1) S.wsum[wv] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int q = 0; q < NWAVES; q++) {
+        const int x = S.wsum[q];
+        before += q < wv ? x : 0;
+        total += x;
+    }
+    int off = before + incl - tsum;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int w = tid * 4 + q;
+        if (w < ncw) S.coff[w] = off;
+        off += cnt[q];
+    }
+    if (tid == 0) S.coff[ncw] = total;
+    __syncthreads();
+    // own cell rows (cells X0 .. X0+cxr-1 of one y, z): particle offsets
+    const int nrows = cyr * czr;
+    if (tid < WAVE) {
+        int oc = 0;
+        if (tid < nrows) {
+            const int wy = s + tid % cyr, wz = DIMS == 3 ? s + tid / cyr : 0;
+            const int w0 = (wz * WY + wy) * WX + s;
+            oc = S.coff[w0 + cxr] - S.coff[w0];
+        }
+        int inc = oc;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int y = __shfl_up(inc, o, WAVE);
+            if (tid >= o) inc += y;
+        }
+        if (tid < nrows) S.orow[tid] = inc - oc;
+        if (tid == WAVE - 1) S.orow[nrows] = inc;
+    }
+    // staging: up to SC_WIN_MAX / SC_TB particles per thread, loads issued
+    // WIN_SB at a time
+    const bool staged = total <= SC_WIN_MAX;
+    if (staged) {
+        const double ox = (double)(X0 - s) * R.wcell, oy = (double)(Y0 - s) * R.wcell;
+        const double oz = DIMS == 3 ? (double)(Z0 - s) * R.wcell : 0.0;
+        for (int a0 = tid; a0 < total; a0 += WIN_SB * SC_TB) {
+            int wc[WIN_SB], jj[WIN_SB];
+            double4 P[WIN_SB];
+#pragma unroll
+            for (int u = 0; u < WIN_SB; u++) {
+                const int a = a0 + u * SC_TB;
+                wc[u] = -1;
+                jj[u] = 0;
+                if (a < total) {
+                    int lo = 0, hi = ncw - 1;  // the window cell holding a: last q with coff[q] <= a
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (S.coff[mid] <= a)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    wc[u] = lo;
+                    jj[u] = S.cglob[lo] + (a - S.coff[lo]);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < WIN_SB; u++)
+                if (wc[u] >= 0) P[u] = ld4(gp(R.pos_alt) + (jj[u]));
+#pragma unroll
+            for (int u = 0; u < WIN_SB; u++) {
+                if (wc[u] < 0) continue;
+                const int a = a0 + u * SC_TB;
+                double sx, sy, sz;
+                (void)win_cell<DIMS>(R, wc[u], WX, WY, X0, Y0, Z0, sx, sy, sz);
+                S.q[a] = make_float4((float)((P[u].x + sx) - ox), (float)((P[u].y + sy) - oy),
+                                     DIMS == 3 ? (float)((P[u].z + sz) - oz) : 0.0f, (float)P[u].w);
+                S.j[a] = jj[u];
+            }
+        }
+    }
+    __syncthreads();
+    const int nown = S.orow[nrows];
+    const int g = tid / LW, l = tid % LW;
+    const unsigned long long gmask = ((1ull << LW) - 1ull) << (lane / LW * LW);
+    const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
+    const float zeta = (float)R.zeta, rcut = (float)R.rcut, skin = (float)R.skin, ljmd = (float)R.lj_md;
+    const float wlo = R.wlo, whi = R.whi, hm = R.hmarg;
+    WinGeo G;
+    G.wf = (float)R.wcell;
+    G.winv = (float)(1.0 / R.wcell);
+    G.eps = 1e-5f * G.wf + 1e-6f;  // > FP32 coordinate error + cell-boundary rounding
+    G.s = s;
+    const float rvm = (float)sqrt(R.Rv2) * 1.00001f + G.eps;
+    const float R2v = rvm * rvm;
+    const float dmax = R.dmax * 1.000001f;
+    const int n = R.n, cap = R.cap;
+    const int side = 2 * s + 1, nst = DIMS == 3 ? side * side : side;  // stencil rows
+    int *rows = S.rows[g];
+    for (int base = 0; base < nown; base += WIN_GROUPS) {  // same trip count for every lane of the block
+        const int o = base + g;
+        const bool active = o < nown;
+        int k = 0, aself = 0, wxk = 0, wy0 = 0, wz0 = 0, T = 0, bound = 0;
+        double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+        float4 pf = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (active) {
+            int r = 0;
+            while (r + 1 < nrows && S.orow[r + 1] <= o) r++;
+            wy0 = s + r % cyr;
+            wz0 = DIMS == 3 ? s + r / cyr : 0;
+            const int w0 = (wz0 * WY + wy0) * WX + s;
+            k = S.cglob[w0] + (o - S.orow[r]);
+            aself = S.coff[w0] + (o - S.orow[r]);
+            if (staged) {
+                p = ld4(gp(R.pos_alt) + (k));
+                int cc[3];
+                cell_of_pos<DIMS>(R, p, cc);
+                wxk = cc[0] - X0 + s;
+                pf = S.q[aself];
+                // largest half-list radius of this particle over every
+                // partner diameter in (0, dmax]: sigma_ij rises with D_j up
+                // to D_j = D_i, and beyond it up to 1/(2 zeta)
+                float smax = ljmd;
+                if (R.pot == 2) {
+                    const float dst = zeta > 0.0f ? fminf(fmaxf(0.5f / zeta, pf.w), dmax) : dmax;
+                    smax = fmaxf(pf.w, (pf.w + dst) * 0.5f * (1.0f - zeta * fabsf(pf.w - dst)));
+                }
+                const float rhm = fminf((rcut * smax + skin) * 1.00001f + G.eps, rvm);
+                const float R2h = rhm * rhm;
+                // this lane's block of stencil rows: candidate counts within
+                // the half-list radius (and LDS starts) and the R_v bound
+                const int B = (nst + LW - 1) / LW;  // rows per lane, lane l takes [l B, l B + B)
+                int ra[WIN_RPL], rn[WIN_RPL], mine = 0;
+#pragma unroll
+                for (int q = 0; q < WIN_RPL; q++) {
+                    const int rr = l * B + q;
+                    ra[q] = 0;
+                    rn[q] = 0;
+                    if (q < B && rr < nst) {
+                        const int dy = rr % side - s, dz = DIMS == 3 ? rr / side - s : 0;
+                        const float gy = cell_gap(wy0 + dy, pf.y, G.wf, G.eps);
+                        const float gz = DIMS == 3 ? cell_gap(wz0 + dz, pf.z, G.wf, G.eps) : 0.0f;
+                        const float g2 = gy * gy + gz * gz;
+                        const int rowc = ((wz0 + dz) * WY + (wy0 + dy)) * WX;
+                        int lo, hi;
+                        win_row_range(G, pf, wxk, g2, R2v, lo, hi);
+                        if (lo <= hi) bound += S.coff[rowc + hi + 1] - S.coff[rowc + lo];
+                        win_row_range(G, pf, wxk, g2, R2h, lo, hi);
+                        if (lo <= hi) {
+                            const int a0 = S.coff[rowc + lo], na = S.coff[rowc + hi + 1] - a0;
+                            if (na > 0) {
+                                ra[q] = a0;
+                                rn[q] = na;
+                            }
+                        }
+                    }
+                    mine += rn[q];
+                }
+                // exclusive scan of the lanes' counts within the group
+                int inc = mine;
+#pragma unroll
+                for (int d = 1; d < LW; d <<= 1) {
+                    const int y = __shfl_up(inc, d, LW);
+                    if (l >= d) inc += y;
+                }
+                T = __shfl(inc, LW - 1, LW);
+                int P = inc - mine;
+#pragma unroll
+                for (int q = 0; q < WIN_RPL; q++) {
+                    const int rr = l * B + q;
+                    if (q < B && rr < nst) rows[rr] = ra[q] | (P << WIN_ABITS);
+                    P += rn[q];
+                }
+                if (l == LW - 1) rows[nst] = T << WIN_ABITS;
+#pragma unroll
+                for (int d = 1; d < LW; d <<= 1) bound += __shfl_xor(bound, d, LW);
+            }
+        }
+        __syncthreads();
+        if (active) {
+            if (!staged) {
+                build_slot_lanes<DIMS, LW>(R, k, l, gmask, below);
+            } else {
+                int cnt_h = 0, r = 0;
+                for (int c0 = 0; c0 < T; c0 += LW) {
+                    const int c = c0 + l;
+                    bool inH = false;
+                    int a = -1;
+                    if (c < T) {
+                        while ((rows[r + 1] >> WIN_ABITS) <= c) r++;
+                        const int e = rows[r];
+                        a = (e & (SC_WIN_MAX - 1)) + (c - (e >> WIN_ABITS));
+                        if (a != aself) {
+                            const float4 q = S.q[a];
+                            const float ex = pf.x - q.x, ey = pf.y - q.y;
+                            float d2 = ex * ex + ey * ey;
+                            if (DIMS == 3) {
+                                const float ez = pf.z - q.z;
+                                d2 = d2 + ez * ez;
+                            }
+                            if (d2 <= whi) {
+                                const float md = (R.pot == 2) ? (pf.w + q.w) * 0.5f * (1.0f - zeta * fabsf(pf.w - q.w))
+                                                              : ljmd;
+                                const float rh = rcut * md + skin;
+                                const float rh2 = rh * rh;
+                                if (d2 < rh2 - hm && d2 <= wlo) {
+                                    inH = true;
+                                } else if (d2 < rh2 + hm) {
+                                    bool inF;
+                                    win_exact<DIMS>(R, p, S.j[a], inF, inH);
+                                }
+                            }
+                        }
+                    }
+                    const unsigned long long mH = __ballot(inH) & gmask;
+                    if (inH) {
+                        const int slot = cnt_h + __popcll(mH & below);
+                        if (slot < cap) gp(R.nbr)[(size_t)slot * n + k] = S.j[a];
+                    }
+                    cnt_h += __popcll(mH);
+                }
+                // The full-list length matters only against maxneighbours:
+                // the candidate count within R_v bounds it; count exactly only
+                // when the bound exceeds the capacity.
+                int nfull = bound;
+                if (bound > cap) {
+                    nfull = 0;
+                    for (int rr = 0; rr < nst; rr++) {
+                        const int dy = rr % side - s, dz = DIMS == 3 ? rr / side - s : 0;
+                        const float gy = cell_gap(wy0 + dy, pf.y, G.wf, G.eps);
+                        const float gz = DIMS == 3 ? cell_gap(wz0 + dz, pf.z, G.wf, G.eps) : 0.0f;
+                        int lo, hi;
+                        win_row_range(G, pf, wxk, gy * gy + gz * gz, R2v, lo, hi);
+                        if (lo > hi) continue;
+                        const int rowc = ((wz0 + dz) * WY + (wy0 + dy)) * WX;
+                        const int a1 = S.coff[rowc + hi + 1];
+                        for (int b = S.coff[rowc + lo]; b < a1; b += LW) {
+                            const int a = b + l;
+                            bool inF = false;
+                            if (a < a1 && a != aself) {
+                                const float4 q = S.q[a];
+                                const float ex = pf.x - q.x, ey = pf.y - q.y;
+                                float d2 = ex * ex + ey * ey;
+                                if (DIMS == 3) {
+                                    const float ez = pf.z - q.z;
+                                    d2 = d2 + ez * ez;
+                                }
+                                if (d2 <= wlo) {
+                                    inF = true;
+                                } else if (d2 <= whi) {
+                                    bool inH2;
+                                    win_exact<DIMS>(R, p, S.j[a], inF, inH2);
+                                }
+                            }
+                            nfull += __popcll(__ballot(inF) & gmask);
+                        }
+                    }
+                }
+                if (l == 0) build_finish_slot<DIMS>(R, k, p, nfull, cnt_h);
+            }
+        }
+        __syncthreads();  // the group's row table is rewritten by the next own particle
+    }
+}
+
+// R5 (windowed form) over the planned (replica, run) pairs, or, with no plan
+// (a single replica), all of the replica's runs when its flag is set.
+__global__ __launch_bounds__(SC_TB, 3) void k_build_win(const Rep *reps, const int *plan, int nreps, int ntiles)
+{
+    __shared__ WinLds S;
+    const int *rl = plan ? plan + 4 + nreps + ntiles : nullptr;
+    const int count = plan ? plan[2]
+                           : ((reps[0].build_kind == BUILD_WIN && gp(reps[0].dyn)->rebuild != 0) ? reps[0].nruns : 0);
+    for (XcdRange it = xcd_range(count); it.cur < it.end; it.cur += it.step) {
+        const int r = rl ? rl[2 * it.cur] : 0;
+        const int run = rl ? rl[2 * it.cur + 1] : it.cur;
+        const Rep R = reps[r];
+        if (R.dims == 3)
+            win_run<3>(R, run, S);
+        else
+            win_run<2>(R, run, S);
+    }
+}
+
 // Full rows at the current positions over the rebuild-time cells, for the
 // reference-layout list export.
 template <int DIMS>
@@ -1356,7 +1819,7 @@ __device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *coun
 {
     const int n = R.n, k = t.first + threadIdx.x;
     if (k >= n) return;
-    const double4 p = R.pos[k];
+    const double4 p = ld4(gp(R.pos) + (k));
     int cc[3];
     cell_of_pos<DIMS>(R, p, cc);
     const int s = R.srange;
@@ -1370,14 +1833,14 @@ __device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *coun
                 for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
                     if (j == k) continue;
                     double dx, dy, dz;
-                    double d2 = sqdist<DIMS>(R, p, R.pos[j], dx, dy, dz);
+                    double d2 = sqdist<DIMS>(R, p, ld4(gp(R.pos) + (j)), dx, dy, dz);
                     if (d2 <= R.Rv2) {
-                        if (cnt < cap) rows[(size_t)R.orig[k] * cap + cnt] = R.orig[j];
+                        if (cnt < cap) rows[(size_t)gp(R.orig)[k] * cap + cnt] = gp(R.orig)[j];
                         cnt++;
                     }
                 }
             }
-    counts[R.orig[k]] = cnt;
+    counts[gp(R.orig)[k]] = cnt;
 }
 
 __global__ __launch_bounds__(SC_TB) void k_full_rows(const Tile *tiles, const Rep *reps, int *rows, int *counts, int cap)
@@ -1401,11 +1864,11 @@ __global__ __launch_bounds__(SC_TB) void k_full_rows(const Tile *tiles, const Re
 template <int DIMS, int POT>
 __device__ void energy_tile(const Rep &R, const Tile &t)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n, k = t.first + threadIdx.x;
     double e = 0.0;
     if (k < n) {
-        const double4 p = R.pos[k];
+        const double4 p = ld4(gp(R.pos) + (k));
         int cc[3];
         cell_of_pos<DIMS>(R, p, cc);
         const int s = R.srange;
@@ -1417,7 +1880,7 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
                                   R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
                     for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
                         if (j == k) continue;
-                        const double4 q = R.pos[j];
+                        const double4 q = ld4(gp(R.pos) + (j));
                         double dx, dy, dz;
                         double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
                         e += pair_energy<DIMS, POT>(R, d2, p.w, q.w);
@@ -1432,8 +1895,8 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
     if (threadIdx.x == 0) {
         double sum = 0.0, comp = 0.0;
         for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
-        R.part[PS_ESUM * R.ntiles + t.local] = sum;
-        R.part[PS_ECOMP * R.ntiles + t.local] = comp;
+        gp(R.part)[PS_ESUM * R.ntiles + t.local] = sum;
+        gp(R.part)[PS_ECOMP * R.ntiles + t.local] = comp;
     }
 }
 
@@ -1455,14 +1918,14 @@ __global__ __launch_bounds__(SC_TB) void k_energy(const Tile *tiles, const Rep *
 template <int DIMS, int POT>
 __device__ void energy_nn_tile(const Rep &R, const Tile &t)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     __shared__ double4 sp[SC_TB];
     const int n = R.n, k = t.first + threadIdx.x;
-    const double4 p = k < n ? R.pos[k] : make_double4(0, 0, 0, 1);
+    const double4 p = k < n ? ld4(gp(R.pos) + (k)) : make_double4(0, 0, 0, 1);
     double e = 0.0;
     for (int b = 0; b < n; b += SC_TB) {
         __syncthreads();
-        if (b + threadIdx.x < n) sp[threadIdx.x] = R.pos[b + threadIdx.x];
+        if (b + threadIdx.x < n) sp[threadIdx.x] = ld4(gp(R.pos) + (b + threadIdx.x));
         __syncthreads();
         const int m = min(SC_TB, n - b);
         if (k < n)
@@ -1479,8 +1942,8 @@ __device__ void energy_nn_tile(const Rep &R, const Tile &t)
     if (threadIdx.x == 0) {
         double sum = 0.0, comp = 0.0;
         for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
-        R.part[PS_ESUM * R.ntiles + t.local] = sum;
-        R.part[PS_ECOMP * R.ntiles + t.local] = comp;
+        gp(R.part)[PS_ESUM * R.ntiles + t.local] = sum;
+        gp(R.part)[PS_ECOMP * R.ntiles + t.local] = comp;
     }
 }
 
@@ -1503,13 +1966,13 @@ __global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *re
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n, i = t.first + threadIdx.x;
     double e[1] = {0.0};
     if (i < n) {
         double s = 0.0;
         for (int c = 0; c < R.dims; c++) {
-            double vc = R.v[c * n + i];
+            double vc = gp(R.v)[c * n + i];
             s = (c == 0) ? vc * vc : s + vc * vc;
         }
         e[0] = s;
@@ -1527,16 +1990,16 @@ __global__ __launch_bounds__(SC_TB) void k_to_host(const Tile *tiles, const Rep 
     const Rep R = reps[t.rep];
     const int n = R.n, k = t.first + threadIdx.x, d = R.dims;
     if (k >= n) return;
-    const int o = R.orig[k];
+    const int o = gp(R.orig)[k];
     double *st = R.stage;
     switch (field) {
     case FLD_R: {
-        double4 p = R.pos[k];
+        double4 p = ld4(gp(R.pos) + (k));
         st[(size_t)o * d + 0] = p.x;
         st[(size_t)o * d + 1] = p.y;
         if (d == 3) st[(size_t)o * d + 2] = p.z;
     } break;
-    case FLD_D: st[o] = R.pos[k].w; break;
+    case FLD_D: st[o] = ld4(gp(R.pos) + (k)).w; break;
     case FLD_V:
     case FLD_F:
     case FLD_DISP: {
@@ -1546,7 +2009,7 @@ __global__ __launch_bounds__(SC_TB) void k_to_host(const Tile *tiles, const Rep 
     case FLD_DISP2: {
         double s = 0.0;
         for (int c = 0; c < d; c++) {
-            double x = R.disp[c * n + k];
+            double x = gp(R.disp)[c * n + k];
             s = (c == 0) ? x * x : s + x * x;
         }
         st[o] = s;
@@ -1560,20 +2023,20 @@ __global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Re
     const Rep R = reps[t.rep];
     const int n = R.n, k = t.first + threadIdx.x, d = R.dims;
     if (k >= n) return;
-    const int o = R.orig[k];
+    const int o = gp(R.orig)[k];
     const double *st = R.stage;
     switch (field) {
     case FLD_R: {
-        double4 p = R.pos[k];
+        double4 p = ld4(gp(R.pos) + (k));
         p.x = st[(size_t)o * d + 0];
         p.y = st[(size_t)o * d + 1];
         p.z = d == 3 ? st[(size_t)o * d + 2] : 0.0;
-        R.pos[k] = p;
+        st4(gp(R.pos) + (k), p);
     } break;
     case FLD_D: {
-        double4 p = R.pos[k];
+        double4 p = ld4(gp(R.pos) + (k));
         p.w = st[o];
-        R.pos[k] = p;
+        st4(gp(R.pos) + (k), p);
     } break;
     case FLD_V:
     case FLD_F:
@@ -1592,13 +2055,13 @@ __global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Re
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n, i = t.first + threadIdx.x;
     double st = 0.0, fm = 0.0;
     if (i < n) {
         double s = 0.0, f2 = 0.0;
         for (int c = 0; c < R.dims; c++) {
-            double fc = R.F[c * n + i];
+            double fc = gp(R.F)[c * n + i];
             double vc = fc * dt;
             s = (c == 0) ? vc * vc : s + vc * vc;
             f2 = (c == 0) ? fc * fc : f2 + fc * fc;
@@ -1615,31 +2078,31 @@ __global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Re
 template <int DIMS>
 __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
 {
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n, i = t.first + threadIdx.x;
     const double scale = dyn->result[2];
     double s2 = 0.0;
     if (i < n) {
-        double4 p = R.pos[i];
+        double4 p = ld4(gp(R.pos) + (i));
         double pc[3] = {p.x, p.y, p.z};
         double s = 0.0;
         bool fin = true;
 #pragma unroll
         for (int c = 0; c < DIMS; c++) {
-            double vc = R.F[c * n + i] * dt;
+            double vc = gp(R.F)[c * n + i] * dt;
             if (scale >= 0.0) vc = vc * scale;
-            R.v[c * n + i] = vc;
-            double dc = R.disp[c * n + i] + vc;
+            gp(R.v)[c * n + i] = vc;
+            double dc = gp(R.disp)[c * n + i] + vc;
             s = (c == 0) ? dc * dc : s + dc * dc;
             double rn = pc[c] + vc;
             rn -= floor(rn / R.L) * R.L;
             pc[c] = rn;
             fin = fin && isfinite(rn);
-            R.disp[c * n + i] = dc;
+            gp(R.disp)[c * n + i] = dc;
         }
-        R.pos[i] = make_double4(pc[0], pc[1], pc[2], p.w);
+        st4(gp(R.pos) + (i), make_double4(pc[0], pc[1], pc[2], p.w));
         s2 = s;
-        if (!fin) atomicOr(&dyn->err, SC_EBIT_NONFINITE);
+        if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }
     publish_maxdisp(R, block_max(s2));
 }
@@ -1678,9 +2141,9 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     if (r >= nreps) return;
     const Rep R = reps[r];
     if (!((sysmask >> R.sys) & 1)) return;
-    RepDyn *dyn = R.dyn;
+    SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int tid = threadIdx.x;
-    const double *P = R.part;
+    const SC_AS1 double *P = gp(R.part);
     switch (op) {
     case FIN_DRIFT:
         break;
@@ -1885,7 +2348,7 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     const int *plan = nreps == 1 ? nullptr : g.plan;
     if (nreps > 1 && !plan) return -1;
     const int gt = ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP;
-    if (plan) hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, g.plan);
+    if (plan) hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, ntiles, g.plan);
     hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
     hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan);
     hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan);
@@ -1894,6 +2357,10 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     if (g.nruns > 0) {
         const int gr = g.nruns < SC_RUN_GRID_CAP ? g.nruns : SC_RUN_GRID_CAP;
         hipLaunchKernelGGL(k_build_cells, dim3(gr), dim3(SC_TB), 0, s, reps, nreps, g.nruns);
+    }
+    if (g.nwin > 0) {
+        const int gw = g.nwin < SC_RUN_GRID_CAP ? g.nwin : SC_RUN_GRID_CAP;
+        hipLaunchKernelGGL(k_build_win, dim3(gw), dim3(SC_TB), 0, s, reps, plan, nreps, ntiles);
     }
     if (g.any_tile_build) {
         const int key = g.build_bs * 16 + g.build_lpp;

@@ -21,7 +21,7 @@ bool params_valid(const sc_params *p)
              p->system < 0 || p->system > 2 || (p->precision != SC_PREC_FP64 && p->precision != SC_PREC_MIXED) ||
              p->N < 2 || p->N > (int64_t)1 << 30 || p->maxneighbours < 1 || p->maxneighbours > 100000 ||
              !(p->box_size > 0.0) || !(p->skin_distance >= 0.0) || !(p->force_cutoff > 0.0) ||
-             p->cell_divisor < 0 || p->cell_divisor > 4 || p->build_mode < 0 || p->build_mode > 1);
+             p->cell_divisor < 0 || p->cell_divisor > 4 || p->build_mode < 0 || p->build_mode > 3);
 }
 
 // Cell grid of a replica: cells of side >= R_v / k with a (2k+1)^dims
@@ -79,6 +79,56 @@ static int A(std::vector<void *> &allocs, T **ptr, size_t count, std::string &er
     return alloc_into(allocs, reinterpret_cast<void **>(ptr), count * sizeof(T), err);
 }
 
+// Windowed build runs: cx x cy x cz own cells (cz = 1 in 2D) whose window
+// (own cells plus srange cells on every side) holds about 1200 particles on
+// average, well inside the SC_WIN_MAX LDS slots (a denser window falls back
+// to global-memory candidates).  Among the shapes that fit, the one with the
+// most own cells per window cell (fewest staged loads per particle) wins.
+//
+// FP32 prefilter margins: window-relative coordinates are below
+// xmax = (largest window side + 1) cells, so each rounded coordinate is off
+// by <= xmax 2^-24 and a separation component by e1 <= 2 xmax 2^-24 +
+// R_v 2^-24; near R_v the FP32 d^2 is then within 3 (2 R_v e1 + e1^2) +
+// 4 R_v^2 2^-24 of the FP64 d^2.  Four times that bound is used, and the
+// half-list radius (FP32 sigma_ij) gets a further 4 R_v^2 2^-20.
+static void window_geometry(Rep &h, int dims, double Rv)
+{
+    const int nc = h.nc, s = h.srange;
+    const double per_cell = (double)h.n / (double)h.ncell;
+    const int lim = std::min(nc, 32);
+    double best = -1.0;
+    int bx = 1, by = 1, bz = 1;
+    for (int cx = 1; cx <= lim; cx++)
+        for (int cy = 1; cy <= lim; cy++)
+            for (int cz = 1; cz <= (dims == 3 ? lim : 1); cz++) {
+                const long long wc = (long long)(cx + 2 * s) * (cy + 2 * s) * (dims == 3 ? cz + 2 * s : 1);
+                if (wc > SC_WIN_CELLS || (long long)cy * cz > SC_WIN_ROWS) continue;
+                if (wc * per_cell > 1200.0 && !(cx == 1 && cy == 1 && cz == 1)) continue;
+                const double own = (double)cx * cy * cz;
+                const double score = own / (double)wc + 1e-6 * own;
+                if (score > best) {
+                    best = score;
+                    bx = cx;
+                    by = cy;
+                    bz = cz;
+                }
+            }
+    h.cx = bx;
+    h.cy = by;
+    h.cz = bz;
+    const long long rx = (nc + bx - 1) / bx, ry = (nc + by - 1) / by, rz = dims == 3 ? (nc + bz - 1) / bz : 1;
+    h.nruns = (int)(rx * ry * rz);
+    const int wmax = std::max(bx, std::max(by, bz)) + 2 * s + 1;
+    const double xmax = wmax * h.wcell;
+    const double u = std::ldexp(1.0, -24);
+    const double e1 = 2.0 * xmax * u + Rv * u + 1e-12;
+    const double Rv2 = Rv * Rv;
+    const double m = 4.0 * (3.0 * (2.0 * Rv * e1 + e1 * e1) + 4.0 * Rv2 * u) + 1e-12;
+    h.wlo = (float)(Rv2 - m);
+    h.whi = (float)(Rv2 + m);
+    h.hmarg = (float)(m + 4.0 * Rv2 * std::ldexp(1.0, -20));
+}
+
 int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *> &allocs, std::string &err)
 {
     std::memset(&h, 0, sizeof(Rep));
@@ -132,9 +182,16 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     h.srange = srange;
     h.cinv = (double)nc / p->box_size;
     h.ntiles = (n + SC_TB - 1) / SC_TB;
-    // LDS build runs: ~64 own particles per block, window <= 64 cells
-    h.lds_build = (srange == 1 && p->build_mode == 1) ? 1 : 0;
-    if (h.lds_build) {
+    // build kind: the windowed LDS build (build_mode 3, stencils up to +-3),
+    // the cell-run LDS build (build_mode 1, stencil +-1), else (automatic and
+    // build_mode 2) the per-slot sweep
+    h.wcell = p->box_size / (double)nc;
+    h.dmax = (float)maxD;
+    if (srange >= 1 && srange <= 3 && p->build_mode == 3) {
+        h.build_kind = BUILD_WIN;
+        window_geometry(h, p->dims, Rv);
+    } else if (srange == 1 && p->build_mode == 1) {
+        h.build_kind = BUILD_CELLS;
         const double per_cell = (double)n / (double)ncell;
         const int cxmax = (p->dims == 3 ? 5 : 19);
         int cx = (int)std::lround(64.0 / std::max(per_cell, 1e-9));
@@ -142,6 +199,8 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
         h.cx = cx;
         const long long rpr = (nc + cx - 1) / cx;
         h.nruns = (int)(rpr * nc * (p->dims == 3 ? nc : 1));
+    } else {
+        h.build_kind = BUILD_SLOT;
     }
     const int d = p->dims;
     const int nchunks = (int)((ncell + SC_CHUNK - 1) / SC_CHUNK);

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from simulationcode_amd import _native as nat
+from simulationcode_amd import _native as nat, synth
 from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
@@ -30,7 +30,10 @@ def case_id(c):
 
 
 @pytest.mark.parametrize("case", CASES, ids=case_id)
-@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (3, 0), (1, 1)])
+# build_mode 0: automatic; 1: LDS cell runs (stencil +-1); 2: per-slot sweep
+# over global memory; 3: LDS windows (FP32 prefilter, FP64 band test)
+@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (3, 0), (1, 1), (1, 2), (2, 2),
+                                                     (1, 3), (2, 3), (3, 3)])
 def test_pair_sets_bit_exact(case, cell_divisor, build_mode):
     kind, dims, N = case
     op, r, D, v, rc = H.make_system(kind, N, dims, seed=100 + N)
@@ -116,13 +119,51 @@ def test_trajectory_rescale_temperature():
     assert np.max(np.abs(g["v"] - st.v)) <= 1e-10
 
 
-@pytest.mark.parametrize("cell_divisor,build_mode", [(2, 0), (1, 1)])
+@pytest.mark.parametrize("cell_divisor,build_mode", [(2, 0), (1, 1), (2, 3)])
 def test_trajectory_many_rebuilds(cell_divisor, build_mode):
     op, st, g, dr, ctx = _trajectory("berthier", 3, 1000, oracle.SYS_LANGEVIN, 400, dt=0.01,
                                       cell_divisor=cell_divisor, build_mode=build_mode)
     assert st.rebuilds[0] >= 5
     assert ctx.rebuild_count() == st.rebuilds[0]
     assert dr <= 1e-10
+
+
+@pytest.mark.parametrize("cell_divisor", [1, 2, 3])
+def test_window_build_lists_equal_per_slot_lists(cell_divisor):
+    """The windowed build writes the per-slot sweep's lists entry for entry
+    (same order), so forces agree bitwise."""
+    op, r, D, v, rc = H.make_system("berthier", 4000, 3, seed=77 + cell_divisor)
+    out = []
+    for mode in (3, 2):
+        ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=mode)
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        out.append((ctx.get_neighbor_lists(), ctx.get_state(r=False, v=False, F=True)["F"]))
+        ctx.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_window_overflow_falls_back_to_global_sweep():
+    """All particles in half of the box: windows in the dense half hold about
+    twice the particles they were sized for (more than fit in LDS) and sweep
+    from global memory; pair sets stay exact."""
+    rng = np.random.default_rng(31)
+    N, dims, L = 8000, 3, 25.0
+    D = synth.berthier_diameters(N, rng)
+    r = rng.uniform(0.0, 1.0, (N, dims)) * np.array([L / 2, L, L])
+    v = np.zeros((N, dims))
+    op = oracle.make_params(dims=dims, N=N, box_size=L, force_cutoff=1.25, skin=0.3, D=D,
+                            potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN, kBT=0.1,
+                            gamma=1.0, dt=0.005, maxneighbours=400)
+    ctx = make_ctx(op, 1.25, r, v, D, cell_divisor=1, build_mode=3)
+    ctx.update_neighbor_lists()
+    nf, full, nh, half = ctx.get_neighbor_lists()
+    lst = oracle.build_lists(op, r, D, cells=True)
+    assert np.array_equal(nf, lst.nfull)
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
+    ctx.close()
 
 
 def test_deterministic_and_replayable():
@@ -160,6 +201,29 @@ def test_overflow_raises():
     ctx = make_ctx(op, rc, r, v, D)
     with pytest.raises(nat.NeighborOverflowError):
         ctx.update_neighbor_lists()
+
+
+@pytest.mark.parametrize("cell_divisor", [1, 2])
+def test_overflow_threshold_exact(cell_divisor):
+    """maxneighbours equal to the longest full row builds; one less is an
+    overflow (the windowed build counts full rows exactly whenever its
+    candidate bound exceeds the capacity)."""
+    op, r, D, v, rc = H.make_system("berthier", 4000, 3, seed=12)
+    M = int(oracle.build_lists(op, r, D, cells=True).nfull.max())
+    for cap, raises in ((M, False), (M - 1, True)):
+        op2 = oracle.make_params(dims=3, N=4000, box_size=op.box_size, force_cutoff=rc, skin=op.skin_distance,
+                                 D=D, potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN,
+                                 kBT=0.15, gamma=1.0, dt=0.005, maxneighbours=cap)
+        ctx = make_ctx(op2, rc, r, v, D, cell_divisor=cell_divisor, build_mode=3)
+        if raises:
+            with pytest.raises(nat.NeighborOverflowError):
+                ctx.update_neighbor_lists()
+        else:
+            ctx.update_neighbor_lists()
+            nf, full, nh, half = ctx.get_neighbor_lists()
+            lst = oracle.build_lists(op2, r, D, cells=True)
+            assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
+        ctx.close()
 
 
 def test_nonfinite_raises():
