@@ -1262,11 +1262,25 @@ __device__ inline void build_range(const Rep &R, int k, const double4 p, int &nf
         if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, ld4(gp(R.pos_alt) + (j)), sx, sy, sz);
 }
 
-// The stencil of slot k's cell as (2s+1)^(dims-1) rows of 2s+1 cells along
-// x.  A row's cells are adjacent cell ids, hence one contiguous slot range,
-// split in two where the row wraps the box (each part with its own image
-// shift); the parts come in the order of a cell-by-cell sweep.  seg(b, e,
-// sx, sy, sz) is called once per part.
+// The stencil of slot k's cell as (2s+1)^(dims-1) rows of cells along x.
+// A row's cells are adjacent cell ids, hence one contiguous slot range,
+// split where the row wraps the box (each part with its own image shift);
+// the parts come in the order of a cell-by-cell sweep.  seg(b, e, sx, sy,
+// sz) is called once per part.
+// Each row keeps only the cells that reach within R_v of the particle: in
+// cell units the particle sits at f in [0, 1] of its cell, so a cell at
+// offset o is at least max(0, o - f, f - (o + 1)) away along that axis; rows
+// whose y/z gap exceeds R_v are skipped and the x run is cut to
+// [floor(f_x - r_x), floor(f_x + r_x)], r_x^2 = R_v^2 - gap^2 (with a 1e-9
+// slack far above the rounding of these quantities).  Only candidates
+// farther than R_v -- which no predicate accepts -- are dropped, and the
+// order of the rest is unchanged.
+__device__ inline double cell_gap64(int o, double f)
+{
+    const double g = o > 0 ? (double)o - f : (o < 0 ? f - (double)(o + 1) : 0.0);
+    return g > 1e-9 ? g - 1e-9 : 0.0;
+}
+
 template <int DIMS, class Seg>
 __device__ inline void for_stencil_rows(const Rep &R, const double4 p, Seg &&seg)
 {
@@ -1274,23 +1288,30 @@ __device__ inline void for_stencil_rows(const Rep &R, const double4 p, Seg &&seg
     cell_of_pos<DIMS>(R, p, cc);
     const int s = R.srange, nc = R.nc;
     const int zr = DIMS == 3 ? s : 0;
+    const double fx = p.x * R.cinv - (double)cc[0], fy = p.y * R.cinv - (double)cc[1];
+    const double fz = DIMS == 3 ? p.z * R.cinv - (double)cc[2] : 0.0;
+    const double rc2 = R.Rv2 * R.cinv * R.cinv * (1.0 + 1e-9) + 1e-9;  // R_v^2 in cell units
     for (int oz = -zr; oz <= zr; oz++) {
         double sz = 0.0;
         const int cz = DIMS == 3 ? wrap_shift(cc[2] + oz, nc, R.L, sz) : 0;
+        const double gz = DIMS == 3 ? cell_gap64(oz, fz) : 0.0;
         for (int oy = -s; oy <= s; oy++) {
+            const double gy = cell_gap64(oy, fy);
+            const double g2 = gy * gy + gz * gz;
+            if (g2 > rc2) continue;
+            const double rx = sqrt(rc2 - g2) + 1e-9;
+            const int lo = max(-s, (int)floor(fx - rx)), hi = min(s, (int)floor(fx + rx));
+            if (lo > hi) continue;
             double sy;
             const int cy = wrap_shift(cc[1] + oy, nc, R.L, sy);
             const int row = nc * (cy + nc * cz);
-            int x0 = cc[0] - s;
-            const int x1 = cc[0] + s;
-            if (x0 < 0) {  // cells nc+x0 .. nc-1, seen across the low face
-                seg(cstart(R, row + x0 + nc), cstart(R, row + nc), -R.L, sy, sz);
-                x0 = 0;
-            }
-            const int xe = x1 < nc ? x1 : nc - 1;
-            seg(cstart(R, row + x0), cstart(R, row + xe + 1), 0.0, sy, sz);
-            if (x1 >= nc)  // cells 0 .. x1-nc, seen across the high face
-                seg(cstart(R, row), cstart(R, row + x1 - nc + 1), R.L, sy, sz);
+            const int x0 = cc[0] + lo, x1 = cc[0] + hi;
+            if (x0 < 0)  // cells seen across the low face
+                seg(cstart(R, row + x0 + nc), cstart(R, row + min(x1, -1) + nc + 1), -R.L, sy, sz);
+            const int b0 = max(x0, 0), b1 = min(x1, nc - 1);
+            if (b0 <= b1) seg(cstart(R, row + b0), cstart(R, row + b1 + 1), 0.0, sy, sz);
+            if (x1 >= nc)  // cells seen across the high face
+                seg(cstart(R, row + max(x0, nc) - nc), cstart(R, row + x1 - nc + 1), R.L, sy, sz);
         }
     }
 }
