@@ -124,6 +124,24 @@ int sc_kinetic_energy(sc_ctx *ctx, double *out);
 /* rescale_velocities! -- src/MD.jl:391-402. */
 int sc_rescale_velocities(sc_ctx *ctx);
 
+/* arrays.r_old_array .= arrays.r_array -- src/MD.jl:682 (start of
+ * perform_molecular_dynamics!): the reference configuration of
+ * sc_f2_self, kept per original particle index. */
+int sc_set_reference_positions(sc_ctx *ctx);
+/* calculate_F2_self! -- src/AuxiliaryFunctions.jl:968-987: mean over
+ * particles of cos(kx x_old) cos(kx x) + sin(kx x_old) sin(kx x) on the x
+ * components (the reference uses kx = 7.2); SC_ERR_STATE before
+ * sc_set_reference_positions. */
+int sc_f2_self(sc_ctx *ctx, double kx, double *F2s);
+/* calculate_ql (averaged = 0) -- src/AuxiliaryFunctions.jl:687-711 with
+ * calculate_qlm :758-801 -- and calculate_ql_averaged (averaged = 1,
+ * :832-902): the Steinhardt order parameter of order l (1..8), averaged
+ * over particles, from neighbours within q_cutoff (at most the Verlet
+ * radius: the reference takes them from the full list), with the
+ * reference's angles (theta = atan(sqrt(x^2+y^2)/z), phi = atan(y/x)).
+ * 3D only (the reference asserts dims == 3): SC_ERR_ARG otherwise. */
+int sc_steinhardt(sc_ctx *ctx, int32_t l, double q_cutoff, int32_t averaged, double *ql);
+
 /* OutputStruct.N_neighbor_list_rebuilds (src/DataStructs.jl:299): read and
  * optionally overwrite (pass set >= 0). */
 int sc_rebuild_count(sc_ctx *ctx, int64_t *out, int64_t set);

@@ -102,6 +102,12 @@ def lib():
         L.or_do_time_steps.argtypes = [P, ctypes.c_int64, ctypes.c_int] + [dp] * 11 + [dp]
         L.or_update_neighbor_lists.restype = ctypes.c_int
         L.or_update_neighbor_lists.argtypes = [P, ctypes.c_int] + [dp] * 8 + [dp]
+        L.or_f2_self.argtypes = [P, dp, dp, ctypes.c_double]
+        L.or_f2_self.restype = ctypes.c_double
+        L.or_ylm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp]
+        for f in (L.or_ql, L.or_ql_averaged):
+            f.argtypes = [P, dp, dp, dp, ctypes.c_int, ctypes.c_double]
+            f.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -255,3 +261,26 @@ class State:
 
     def full_energy(self):
         return full_energy(self.p, self.r, self.D, self.lists)
+
+
+def f2_self(p, r_old, r, kx=7.2):
+    """calculate_F2_self! (src/AuxiliaryFunctions.jl:968-987)."""
+    r_old = np.ascontiguousarray(r_old, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    return lib().or_f2_self(ctypes.byref(p), _ptr(r_old), _ptr(r), kx)
+
+
+def ylm(l, m, theta, phi):
+    """Y_l^m(theta, phi) as the oracle evaluates it (complex)."""
+    re = np.zeros(1)
+    im = np.zeros(1)
+    lib().or_ylm(int(l), int(m), float(theta), float(phi), _ptr(re), _ptr(im))
+    return complex(re[0], im[0])
+
+
+def steinhardt(p, r, lists, l, q_cutoff, averaged=False):
+    """calculate_ql / calculate_ql_averaged (src/AuxiliaryFunctions.jl:687-902)
+    over the oracle's full lists."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    f = lib().or_ql_averaged if averaged else lib().or_ql
+    return f(ctypes.byref(p), _ptr(r), _ptr(lists.nfull), _ptr(lists.full), int(l), float(q_cutoff))

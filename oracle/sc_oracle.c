@@ -37,6 +37,8 @@
 #define OR_SYS_LANGEVIN 1
 #define OR_SYS_BROWNIAN 2
 
+#define OR_PI 3.141592653589793 /* Float64(pi) */
+
 #define OR_OK 0
 #define OR_ERR_OVERFLOW -1
 #define OR_ERR_ARG -2
@@ -535,4 +537,158 @@ int or_update_neighbor_lists(const or_params *p, int use_cells, const double *r,
                              double *disp, double *disp2, int64_t *rebuilds)
 {
     return rebuild(p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+}
+
+/* ------------------------------------------------------------------------
+ * Log-step observables
+ * ------------------------------------------------------------------------ */
+
+/* calculate_F2_self! -- src/AuxiliaryFunctions.jl:968-987: rold and r AoS,
+ * x components only, serial sum over particles. */
+double or_f2_self(const or_params *p, const double *rold, const double *r, double kx)
+{
+    int dims = p->dims;
+    double F2s = 0.0;
+    for (int64_t i = 0; i < p->N; i++) {
+        double krold = kx * rold[i * dims];
+        double sinkrold = sin(krold), coskrold = cos(krold);
+        double kr = kx * r[i * dims];
+        double sinkr = sin(kr), coskr = cos(kr);
+        F2s += coskrold * coskr + sinkrold * sinkr;
+    }
+    return F2s / (double)p->N;
+}
+
+/* Y_l^m(theta, phi) of SphericalHarmonics.jl's sphericalharmonic(theta, phi;
+ * l, m) (an unpinned dependency, Project.toml; its published definition):
+ * orthonormal, Condon-Shortley phase, P_l^m(cos theta) built from
+ * P_m^m = (-1)^m (2m-1)!! sin^m(theta) with the signed sine, and
+ * Y_l^{-m} = (-1)^m conj(Y_l^m). */
+void or_ylm(int l, int m, double theta, double phi, double *re, double *im)
+{
+    int am = m < 0 ? -m : m;
+    double x = cos(theta), s = sin(theta);
+    double pmm = 1.0;
+    for (int k = 1; k <= am; k++) pmm *= -(2.0 * k - 1.0) * s;
+    double plm = pmm;
+    if (l > am) {
+        double p0 = pmm, p1 = x * (2.0 * am + 1.0) * pmm;
+        for (int ll = am + 2; ll <= l; ll++) {
+            double p2 = ((2.0 * ll - 1.0) * x * p1 - (ll + am - 1.0) * p0) / (double)(ll - am);
+            p0 = p1;
+            p1 = p2;
+        }
+        plm = p1;
+    }
+    double ratio = 1.0;
+    for (int q = l - am + 1; q <= l + am; q++) ratio /= (double)q;
+    double K = sqrt((2.0 * l + 1.0) / (4.0 * OR_PI) * ratio);
+    double yr = K * plm * cos(am * phi), yi = K * plm * sin(am * phi);
+    if (m < 0) {
+        double sg = (am % 2) ? -1.0 : 1.0;
+        yr = sg * yr;
+        yi = -sg * yi;
+    }
+    *re = yr;
+    *im = yi;
+}
+
+/* calculate_qlm -- src/AuxiliaryFunctions.jl:758-801: mean of Y_l^m over the
+ * full-list neighbours of particle i within q_cutoff; returns their number. */
+static int64_t qlm_one(const or_params *p, const double *r, const int64_t *nfull, const int64_t *full,
+                       int64_t i, int l, int m, double qc2, double *re, double *im)
+{
+    int64_t M = p->maxneighbours;
+    double L = p->box_size;
+    double xi = r[i * 3], yi = r[i * 3 + 1], zi = r[i * 3 + 2];
+    double qr = 0.0, qi = 0.0;
+    int64_t nq = 0;
+    for (int64_t k = 0; k < nfull[i]; k++) {
+        int64_t j = full[i * M + k];
+        double xij = xi - r[j * 3], yij = yi - r[j * 3 + 1], zij = zi - r[j * 3 + 2];
+        xij -= rint(xij / L) * L;
+        yij -= rint(yij / L) * L;
+        zij -= rint(zij / L) * L;
+        if (xij * xij + yij * yij + zij * zij > qc2) continue;
+        nq++;
+        double theta = atan(sqrt(xij * xij + yij * yij) / zij);
+        if (xij == 0.0) xij += 1.0000000000000006e-10; /* 10^-10 = inv(10)^10 in Julia */
+        double phi = atan(yij / xij);
+        double yr, yim;
+        or_ylm(l, m, theta, phi, &yr, &yim);
+        qr += yr;
+        qi += yim;
+    }
+    if (nq > 0) {
+        qr /= (double)nq;
+        qi /= (double)nq;
+    }
+    *re = qr;
+    *im = qi;
+    return nq;
+}
+
+/* calculate_ql -- src/AuxiliaryFunctions.jl:687-711 (3D). */
+double or_ql(const or_params *p, const double *r, const int64_t *nfull, const int64_t *full, int l,
+             double q_cutoff)
+{
+    double qc2 = q_cutoff * q_cutoff, avg = 0.0;
+    for (int64_t i = 0; i < p->N; i++) {
+        double tot = 0.0;
+        for (int m = -l; m <= l; m++) {
+            double re, im;
+            qlm_one(p, r, nfull, full, i, l, m, qc2, &re, &im);
+            double a = hypot(re, im);
+            tot += a * a;
+        }
+        avg += sqrt(4.0 * OR_PI / (2.0 * l + 1.0) * tot);
+    }
+    return avg / (double)p->N;
+}
+
+/* calculate_ql_averaged -- src/AuxiliaryFunctions.jl:832-902 (3D). */
+double or_ql_averaged(const or_params *p, const double *r, const int64_t *nfull, const int64_t *full, int l,
+                      double q_cutoff)
+{
+    int64_t N = p->N, M = p->maxneighbours, W = 2 * l + 1;
+    double L = p->box_size, qc2 = q_cutoff * q_cutoff;
+    double *qre = malloc(sizeof(double) * W * N), *qim = malloc(sizeof(double) * W * N);
+    int64_t *nqa = malloc(sizeof(int64_t) * W * N);
+    if (!qre || !qim || !nqa) {
+        free(qre);
+        free(qim);
+        free(nqa);
+        return NAN;
+    }
+    for (int64_t i = 0; i < N; i++)
+        for (int m = -l; m <= l; m++)
+            nqa[i * W + m + l] = qlm_one(p, r, nfull, full, i, l, m, qc2, &qre[i * W + m + l], &qim[i * W + m + l]);
+    double avg = 0.0;
+    for (int64_t i = 0; i < N; i++) {
+        double xi = r[i * 3], yi = r[i * 3 + 1], zi = r[i * 3 + 2];
+        double tot = 0.0;
+        for (int m = -l; m <= l; m++) {
+            double sr = qre[i * W + m + l], si = qim[i * W + m + l];
+            for (int64_t k = 0; k < nfull[i]; k++) {
+                int64_t j = full[i * M + k];
+                double xij = xi - r[j * 3], yij = yi - r[j * 3 + 1], zij = zi - r[j * 3 + 2];
+                xij -= rint(xij / L) * L;
+                yij -= rint(yij / L) * L;
+                zij -= rint(zij / L) * L;
+                if (xij * xij + yij * yij + zij * zij > qc2) continue;
+                sr += qre[j * W + m + l];
+                si += qim[j * W + m + l];
+            }
+            double den = (double)(nqa[i * W + m + l] + 1);
+            sr /= den;
+            si /= den;
+            double a = hypot(sr, si);
+            tot += a * a;
+        }
+        avg += sqrt(tot * 4.0 * OR_PI / (2.0 * l + 1.0));
+    }
+    free(qre);
+    free(qim);
+    free(nqa);
+    return avg / (double)N;
 }

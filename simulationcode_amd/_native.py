@@ -51,6 +51,7 @@ EXPORTS = (
     "sc_batch_neighbor_stats", "sc_batch_replica_error",
     "sc_set_build_launch", "sc_batch_set_build_launch", "sc_autotune", "sc_batch_autotune",
     "sc_batch_step_async", "sc_batch_wait",
+    "sc_set_reference_positions", "sc_f2_self", "sc_steinhardt",
 )
 
 
@@ -146,6 +147,9 @@ def load(path=LIB_PATH):
         "sc_energy_no_neigh": (ctypes.c_int, [vp, pd]),
         "sc_kinetic_energy": (ctypes.c_int, [vp, pd]),
         "sc_rescale_velocities": (ctypes.c_int, [vp]),
+        "sc_set_reference_positions": (ctypes.c_int, [vp]),
+        "sc_f2_self": (ctypes.c_int, [vp, dbl, pd]),
+        "sc_steinhardt": (ctypes.c_int, [vp, ctypes.c_int32, dbl, ctypes.c_int32, pd]),
         "sc_rebuild_count": (ctypes.c_int, [vp, ctypes.POINTER(i64), i64]),
         "sc_minimize": (ctypes.c_int, [vp, i64, dbl, dbl, pd]),
         "sc_philox4x32_10": (None, [vp, vp, vp]),
@@ -311,6 +315,21 @@ class Context:
 
     def rescale_velocities(self):
         self._check(self._lib.sc_rescale_velocities(self._h))
+
+    def set_reference_positions(self):
+        """r_old := r (the reference configuration of f2_self)."""
+        self._check(self._lib.sc_set_reference_positions(self._h))
+
+    def f2_self(self, kx=7.2):
+        out = ctypes.c_double()
+        self._check(self._lib.sc_f2_self(self._h, float(kx), ctypes.byref(out)))
+        return out.value
+
+    def steinhardt(self, l, q_cutoff, averaged=False):
+        out = ctypes.c_double()
+        self._check(self._lib.sc_steinhardt(self._h, int(l), float(q_cutoff), 1 if averaged else 0,
+                                            ctypes.byref(out)))
+        return out.value
 
     def rebuild_count(self, set_to=-1):
         out = ctypes.c_int64()

@@ -35,6 +35,9 @@ struct sc_ctx {
     bool timing = false;
     std::vector<hipEvent_t> kev;
     sc::RebuildGeom geom{};
+    double *d_rold = nullptr;  // reference configuration (original order), sc_set_reference_positions
+    double *d_qlm = nullptr;   // Steinhardt scratch: 2 (QL_LMAX+1) doubles and one int per particle
+    int *d_nq = nullptr;
     double t_force = 0.0, t_drift = 0.0;
     long long n_force = 0, n_drift = 0;
 };
@@ -405,6 +408,51 @@ int sc_energy_no_neigh(sc_ctx *c, double *out)
     RepDyn d;
     int rc = read_dyn(c, &d);
     if (rc) return rc;
+    *out = d.result[0];
+    return SC_OK;
+}
+
+int sc_set_reference_positions(sc_ctx *c)
+{
+    if (!c || !c->d_rep) return SC_ERR_ARG;
+    int rc;
+    if (!c->d_rold && (rc = dalloc(c, &c->d_rold, (size_t)3 * c->h.n))) return rc;
+    KCHK(c, sc::launch_set_rold(c->d_tiles, c->ntiles, c->d_rep, c->d_rold, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
+int sc_f2_self(sc_ctx *c, double kx, double *out)
+{
+    if (!c || !c->d_rep || !out) return SC_ERR_ARG;
+    if (!c->d_rold) return fail(c, SC_ERR_STATE, "no reference configuration (sc_set_reference_positions)");
+    KCHK(c, sc::launch_f2s(c->d_tiles, c->ntiles, c->d_rep, 1, c->d_rold, kx, c->s));
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    *out = d.result[0];
+    return SC_OK;
+}
+
+int sc_steinhardt(sc_ctx *c, int32_t l, double q_cutoff, int32_t averaged, double *out)
+{
+    if (!c || !c->d_rep || !out) return SC_ERR_ARG;
+    if (c->h.dims != 3) return fail(c, SC_ERR_ARG, "Steinhardt order parameters are defined for dims == 3");
+    if (l < 1 || l > sc::QL_LMAX) return fail(c, SC_ERR_ARG, "Steinhardt order l must be 1.." + std::to_string(sc::QL_LMAX));
+    if (!(q_cutoff >= 0.0)) return fail(c, SC_ERR_ARG, "q_cutoff must be non-negative");
+    // the reference's neighbours come from the full list, i.e. within R_v
+    const double qc2 = std::min(q_cutoff * q_cutoff, c->h.Rv2);
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    if (!c->d_qlm) {
+        if ((rc = dalloc(c, &c->d_qlm, (size_t)2 * (sc::QL_LMAX + 1) * c->h.n))) return rc;
+        if ((rc = dalloc(c, &c->d_nq, (size_t)c->h.n))) return rc;
+    }
+    KCHK(c, sc::launch_steinhardt(c->d_tiles, c->ntiles, c->d_rep, 1, l, qc2, averaged ? 1 : 0,
+                                  c->d_qlm, c->d_nq, c->s));
+    if ((rc = read_dyn(c, &d))) return rc;
     *out = d.result[0];
     return SC_OK;
 }

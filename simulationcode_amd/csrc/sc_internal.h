@@ -137,6 +137,12 @@ int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, h
 int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int apply_rescale, hipStream_t s);
+int launch_set_rold(const Tile *tiles, int ntiles, const Rep *reps, double *rold, hipStream_t s);
+int launch_f2s(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const double *rold, double kx,
+               hipStream_t s);
+constexpr int QL_LMAX = 8;  // largest Steinhardt order l served
+int launch_steinhardt(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int l, double qc2, int averaged,
+                      double *qlm, int *nq, hipStream_t s);
 int launch_from_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
 int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
 int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, int *counts, int cap, hipStream_t s);

@@ -2003,6 +2003,212 @@ __global__ __launch_bounds__(SC_TB) void k_ekin(const Tile *tiles, const Rep *re
 }
 
 // ====================================================================
+// Log-step observables: calculate_F2_self! and the Steinhardt order
+// parameters (src/AuxiliaryFunctions.jl:687-902, 968-987)
+// ====================================================================
+
+// Compensated tile partial of one value per slot (PS_ESUM / PS_ECOMP).
+__device__ inline void tile_partial_compensated(const Rep &R, const Tile &t, double e)
+{
+    __shared__ double ss[SC_TB];
+    ss[threadIdx.x] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double sum = 0.0, comp = 0.0;
+        for (int q = 0; q < SC_TB; q++) neumaier_add(sum, comp, ss[q]);
+        gp(R.part)[PS_ESUM * R.ntiles + t.local] = sum;
+        gp(R.part)[PS_ECOMP * R.ntiles + t.local] = comp;
+    }
+}
+
+// r_old_array .= r_array (src/MD.jl:682), kept in original particle order
+// (component c of particle o at rold[c n + o]) so that it survives the
+// re-sorting of slots at every rebuild.
+__global__ __launch_bounds__(SC_TB) void k_set_rold(const Tile *tiles, const Rep *reps, double *rold)
+{
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
+    const int n = R.n, k = t.first + threadIdx.x;
+    if (k >= n) return;
+    const double4 p = ld4(gp(R.pos) + k);
+    const int o = gp(R.orig)[k];
+    SC_AS1 double *ro = gp(rold);
+    ro[o] = p.x;
+    ro[n + o] = p.y;
+    if (R.dims == 3) ro[2 * n + o] = p.z;
+}
+
+// calculate_F2_self! (src/AuxiliaryFunctions.jl:968-987): per particle
+// cos(k x_old) cos(k x) + sin(k x_old) sin(k x) on the x components.
+__global__ __launch_bounds__(SC_TB) void k_f2s(const Tile *tiles, const Rep *reps, const double *rold, double kx)
+{
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
+    const int n = R.n, k = t.first + threadIdx.x;
+    double e = 0.0;
+    if (k < n) {
+        const double x = ld4(gp(R.pos) + k).x;
+        const double xo = gp(rold)[gp(R.orig)[k]];
+        double so, co, s, c;
+        sincos(kx * xo, &so, &co);
+        sincos(kx * x, &s, &c);
+        e = co * c + so * s;
+    }
+    tile_partial_compensated(R, t, e);
+}
+
+// Spherical harmonics Y_l^m(theta, phi), m = 0..L, orthonormal with the
+// Condon-Shortley phase: K_l^m P_l^m(cos theta) e^{i m phi} with
+// P_m^m = (-1)^m (2m-1)!! sin^m(theta) (the signed sine, so theta outside
+// [0, pi] continues the formula), the upward recursion
+// (l-m) P_l^m = (2l-1) x P_{l-1}^m - (l+m-1) P_{l-2}^m, and
+// K_l^m = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!).  Added to (re, im).
+template <int L>
+__device__ inline void ylm_add(double theta, double phi, double (&re)[L + 1], double (&im)[L + 1])
+{
+    double s, x, sp, cp;
+    sincos(theta, &s, &x);
+    sincos(phi, &sp, &cp);
+    double pmm = 1.0, cm = 1.0, sm = 0.0;
+#pragma unroll
+    for (int m = 0; m <= L; m++) {
+        if (m > 0) {
+            pmm *= -(2.0 * m - 1.0) * s;
+            const double c2 = cm * cp - sm * sp;
+            sm = sm * cp + cm * sp;
+            cm = c2;
+        }
+        double plm = pmm;
+        if (L > m) {
+            double p0 = pmm, p1 = x * (2.0 * m + 1.0) * pmm;
+#pragma unroll
+            for (int ll = m + 2; ll <= L; ll++) {
+                const double p2 = ((2.0 * ll - 1.0) * x * p1 - (ll + m - 1.0) * p0) / (double)(ll - m);
+                p0 = p1;
+                p1 = p2;
+            }
+            plm = p1;
+        }
+        double ratio = 1.0;
+#pragma unroll
+        for (int q = L - m + 1; q <= L + m; q++) ratio /= (double)q;
+        const double K = sqrt((2.0 * L + 1.0) / (4.0 * M_PI) * ratio);
+        re[m] += K * plm * cm;
+        im[m] += K * plm * sm;
+    }
+}
+
+// Visits every other slot j within the rebuild-time cell stencil of the
+// particle at p whose minimum-image d2 <= qc2, as f(j, dx, dy, dz).
+template <class Fn>
+__device__ inline void qcut_neighbours(const Rep &R, int k, const double4 p, double qc2, Fn &&fn)
+{
+    int cc[3];
+    cell_of_pos<3>(R, p, cc);
+    const int s = R.srange, nc = R.nc;
+    for (int oz = -s; oz <= s; oz++)
+        for (int oy = -s; oy <= s; oy++)
+            for (int ox = -s; ox <= s; ox++) {
+                const int c = wrapc(cc[0] + ox, nc) + nc * (wrapc(cc[1] + oy, nc) + nc * wrapc(cc[2] + oz, nc));
+                for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
+                    if (j == k) continue;
+                    const double4 q = ld4(gp(R.pos) + j);
+                    const double dx = min_image(p.x - q.x, R.L, R.invL);
+                    const double dy = min_image(p.y - q.y, R.L, R.invL);
+                    const double dz = min_image(p.z - q.z, R.L, R.invL);
+                    if (dx * dx + dy * dy + dz * dz > qc2) continue;
+                    fn(j, dx, dy, dz);
+                }
+            }
+}
+
+// The reference's angles (src/AuxiliaryFunctions.jl:785-790): theta =
+// atan(sqrt(x^2+y^2)/z), phi = atan(y/x) after nudging x == 0 by the
+// literal 10^-10 (in Julia inv(10)^10).
+constexpr double QL_NUDGE = 1.0000000000000006e-10;
+
+// calculate_qlm for every m and calculate_ql's per-particle value
+// (src/AuxiliaryFunctions.jl:687-711, 758-801): q_lm = mean over neighbours
+// within q_cutoff of Y_l^m, stored (real at qlm[2m n + k], imaginary at
+// qlm[(2m+1) n + k], count at nq[k]) for the averaged form; the tile
+// partials sum q_l(i) = sqrt(4 pi/(2l+1) sum_{m=-l..l} |q_lm|^2), using
+// |q_{l,-m}| = |q_lm|.
+template <int L>
+__global__ __launch_bounds__(SC_TB) void k_qlm(const Tile *tiles, const Rep *reps, double qc2, double *qlm, int *nq)
+{
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
+    const int n = R.n, k = t.first + threadIdx.x;
+    double e = 0.0;
+    if (k < n) {
+        double re[L + 1], im[L + 1];
+#pragma unroll
+        for (int m = 0; m <= L; m++) re[m] = im[m] = 0.0;
+        int cnt = 0;
+        const double4 p = ld4(gp(R.pos) + k);
+        qcut_neighbours(R, k, p, qc2, [&](int, double dx, double dy, double dz) {
+            cnt++;
+            const double theta = atan(sqrt(dx * dx + dy * dy) / dz);
+            const double phi = atan(dy / (dx == 0.0 ? dx + QL_NUDGE : dx));
+            ylm_add<L>(theta, phi, re, im);
+        });
+        double tot = 0.0;
+#pragma unroll
+        for (int m = 0; m <= L; m++) {
+            if (cnt > 0) {
+                re[m] /= (double)cnt;
+                im[m] /= (double)cnt;
+            }
+            gp(qlm)[(size_t)(2 * m) * n + k] = re[m];
+            gp(qlm)[(size_t)(2 * m + 1) * n + k] = im[m];
+            const double a = hypot(re[m], im[m]);
+            tot += (m == 0 ? 1.0 : 2.0) * (a * a);
+        }
+        gp(nq)[k] = cnt;
+        e = sqrt(4.0 * M_PI / (2.0 * L + 1.0) * tot);
+    }
+    tile_partial_compensated(R, t, e);
+}
+
+// calculate_ql_averaged's second pass (src/AuxiliaryFunctions.jl:866-893):
+// (q_lm(i) + sum over the same neighbours of q_lm(j)) / (N_q(i) + 1).
+template <int L>
+__global__ __launch_bounds__(SC_TB) void k_qlm_avg(const Tile *tiles, const Rep *reps, double qc2,
+                                                  const double *qlm, const int *nq)
+{
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
+    const int n = R.n, k = t.first + threadIdx.x;
+    double e = 0.0;
+    if (k < n) {
+        double re[L + 1], im[L + 1];
+        const SC_AS1 double *Q = gp(qlm);
+#pragma unroll
+        for (int m = 0; m <= L; m++) {
+            re[m] = Q[(size_t)(2 * m) * n + k];
+            im[m] = Q[(size_t)(2 * m + 1) * n + k];
+        }
+        const double4 p = ld4(gp(R.pos) + k);
+        qcut_neighbours(R, k, p, qc2, [&](int j, double, double, double) {
+#pragma unroll
+            for (int m = 0; m <= L; m++) {
+                re[m] += Q[(size_t)(2 * m) * n + j];
+                im[m] += Q[(size_t)(2 * m + 1) * n + j];
+            }
+        });
+        const double den = (double)(gp(nq)[k] + 1);
+        double tot = 0.0;
+#pragma unroll
+        for (int m = 0; m <= L; m++) {
+            const double a = hypot(re[m] / den, im[m] / den);
+            tot += (m == 0 ? 1.0 : 2.0) * (a * a);
+        }
+        e = sqrt(tot * (4.0 * M_PI) / (2.0 * L + 1.0));
+    }
+    tile_partial_compensated(R, t, e);
+}
+
+// ====================================================================
 // Host-order transfers: staging buffer indexed by original particle index.
 // ====================================================================
 __global__ __launch_bounds__(SC_TB) void k_to_host(const Tile *tiles, const Rep *reps, int field)
@@ -2153,7 +2359,8 @@ enum {
     FIN_ENERGY,         // compensated energy total, halved
     FIN_EKIN,           // kinetic energy
     FIN_EKIN_RESCALE,   // + rescale_velocities! factor (pending)
-    FIN_MINSCALE        // minimiser step scale and max |F|
+    FIN_MINSCALE,       // minimiser step scale and max |F|
+    FIN_MEAN            // compensated total / n
 };
 
 __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, int op, int sysmask, double arg)
@@ -2239,6 +2446,16 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
                 comp += P[PS_ECOMP * R.ntiles + q];
             }
             dyn->result[0] = (sum + comp) / 2.0;
+        }
+        break;
+    case FIN_MEAN:
+        if (tid == 0) {
+            double sum = 0.0, comp = 0.0;
+            for (int q = 0; q < R.ntiles; q++) {
+                neumaier_add(sum, comp, P[PS_ESUM * R.ntiles + q]);
+                comp += P[PS_ECOMP * R.ntiles + q];
+            }
+            dyn->result[0] = (sum + comp) / (double)R.n;
         }
         break;
     case FIN_MINSCALE: {
@@ -2421,6 +2638,49 @@ int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, int nreps, 
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_energy_nn, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
     fin(reps, nreps, FIN_ENERGY, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
+}
+
+int launch_set_rold(const Tile *tiles, int ntiles, const Rep *reps, double *rold, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_set_rold, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, rold);
+    return check(hipGetLastError());
+}
+
+int launch_f2s(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const double *rold, double kx,
+               hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_f2s, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, rold, kx);
+    fin(reps, nreps, FIN_MEAN, SYS_ALL, 0.0, s);
+    return check(hipGetLastError());
+}
+
+template <int L>
+static void launch_ql_l(const Tile *tiles, int ntiles, const Rep *reps, double qc2, int averaged, double *qlm,
+                        int *nq, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_qlm<L>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, qc2, qlm, nq);
+    if (averaged) hipLaunchKernelGGL(k_qlm_avg<L>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, qc2, qlm, nq);
+}
+
+int launch_steinhardt(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int l, double qc2, int averaged,
+                      double *qlm, int *nq, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    switch (l) {
+    case 1: launch_ql_l<1>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 2: launch_ql_l<2>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 3: launch_ql_l<3>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 4: launch_ql_l<4>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 5: launch_ql_l<5>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 6: launch_ql_l<6>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 7: launch_ql_l<7>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    case 8: launch_ql_l<8>(tiles, ntiles, reps, qc2, averaged, qlm, nq, s); break;
+    default: return -1;
+    }
+    fin(reps, nreps, FIN_MEAN, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }
 

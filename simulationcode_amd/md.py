@@ -21,7 +21,10 @@ Reference map:
   calculate_full_energy ................ src/AuxiliaryFunctions.jl:22-34
   find_kinetic_energy .................. src/AuxiliaryFunctions.jl:57-65
   calculate_full_energy_no_neigh ....... src/AuxiliaryFunctions.jl:86-110
+  calculate_ql / calculate_ql_averaged . src/AuxiliaryFunctions.jl:687-711, 832-902
   logdata .............................. src/AuxiliaryFunctions.jl:637-646
+  calculate_F2_self .................... src/AuxiliaryFunctions.jl:968-987
+  print_log_data (in the MD loop) ...... src/MD.jl:610-650, crystallisation guard :711-713
   create_when_to_save_array ............ src/Dump.jl:18-31
   find_inherent_structure .............. src/InherentStructure.jl:5-87
 """
@@ -370,6 +373,41 @@ def calculate_full_energy_no_neigh(arrays, parameters, neighborlist):
     return ctx.energy_no_neigh()
 
 
+def calculate_F2_self(arrays, parameters, output, neighborlist):
+    """src/AuxiliaryFunctions.jl:968-987 on the device: arrays.r_old_array is
+    the reference configuration, arrays.r_array the current one (kx = 7.2).
+    (The device path of perform_molecular_dynamics keeps the reference
+    configuration resident and calls the context directly.)"""
+    ctx = _bind(arrays, parameters, neighborlist)
+    ctx.set_state(r=arrays.r_old_array)
+    ctx.set_reference_positions()
+    ctx.set_state(r=arrays.r_array)
+    output.F2s = ctx.f2_self(7.2)
+
+
+def _steinhardt(arrays, parameters, output, neighborlist, l, averaged):
+    ctx = _bind(arrays, parameters, neighborlist)
+    ctx.set_state(r=arrays.r_array)
+    q = ctx.steinhardt(l, parameters.q_cutoff, averaged)
+    if l == 6:
+        output.q6 = q
+    elif l == 4:
+        output.q4 = q
+    else:
+        print(q)
+
+
+def calculate_ql(arrays, parameters, output, neighborlist, l):
+    """src/AuxiliaryFunctions.jl:687-711 (with calculate_qlm :758-801) on the
+    device; 3D only (ScError otherwise, as the reference asserts)."""
+    _steinhardt(arrays, parameters, output, neighborlist, l, False)
+
+
+def calculate_ql_averaged(arrays, parameters, output, neighborlist, l):
+    """src/AuxiliaryFunctions.jl:832-902 on the device."""
+    _steinhardt(arrays, parameters, output, neighborlist, l, True)
+
+
 def find_kinetic_energy(arrays, parameters, output):
     """src/AuxiliaryFunctions.jl:57-65."""
     v = arrays.v_array
@@ -459,6 +497,7 @@ def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, rest
         output.steps_done = 0
     update_neighbor_lists(arrays, parameters, output, neighborlist)
     ctx = neighborlist.ctx
+    ctx.set_reference_positions()  # r_old_array on the device (positions unchanged since :682)
     if equilibrate_velocities:
         run_short_MD_equilibration(arrays, parameters, output, neighborlist, noise)
         rescale_velocities(arrays.v_array, parameters)
@@ -497,12 +536,25 @@ def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, rest
             _upload(arrays, neighborlist, ctx)
         save_data(arrays, parameters, output, neighborlist, restarted, writer)
         if logdata(output.steps_done):
+            # print_log_data (src/MD.jl:610-650) with its observables on the device
             output.potential_energy = ctx.full_energy()
             if not isinstance(S, Brownian):
                 find_kinetic_energy(arrays, parameters, output)
+            else:
+                output.kinetic_energy = 0.0
+            if parameters.q_cutoff > 0.0:
+                output.q6 = ctx.steinhardt(6, parameters.q_cutoff, True)
+                output.q4 = ctx.steinhardt(4, parameters.q_cutoff, True)
+            output.F2s = ctx.f2_self(7.2)
             if log is not None:
-                log(f"{output.steps_done}/{parameters.N_steps}, E_pot = {output.potential_energy:.12f}, "
-                    f"E_kin = {output.kinetic_energy:.4f}, elapsed = {time.time() - start:.2f}s")
+                kT = output.kinetic_energy * 2 / S.dims / parameters.N
+                qs = f"q4 = {output.q4:.3f}, q6 = {output.q6:.3f}, " if S.dims == 3 else ""
+                log(f"{output.steps_done}/{parameters.N_steps}, "
+                    f"E = {output.potential_energy + output.kinetic_energy:.2f}, "
+                    f"E_pot = {output.potential_energy:.12f}, E_kin = {output.kinetic_energy:.2f}, "
+                    f"kT = {kT:.3f}, {qs}F2s = {output.F2s:.3f}, elapsed = {time.time() - start:.2f}s")
+            if output.q6 > 0.2:  # src/MD.jl:711-713
+                raise RuntimeError("Simulation has crystallized, terminating...")
     if log is not None:
         log(f"Number of neighbor lists built = {output.N_neighbor_list_rebuilds}")
 
