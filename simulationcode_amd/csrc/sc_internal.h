@@ -118,6 +118,7 @@ struct RebuildGeom {
     int nwin = 0;            // windowed-build runs over all replicas (BUILD_WIN)
     int any_tile_build = 0;  // some replica uses the per-slot build (all-pairs / explicit)
     int *plan = nullptr;     // device work list, plan_ints(nreps, ntiles, nwin) ints (k_rebuild_plan)
+    const int *flag = nullptr;  // one replica without a plan: its RepDyn::rebuild on the device
     int build_bs = 256;      // per-slot build launch shape: threads per block (64 or 256)
     int build_lpp = 1;       //   and lanes per slot (1, 2, 4 or 8); the lists do not depend on it
 };

@@ -810,17 +810,19 @@ struct RebuildWork {
     __device__ int tile(int w) const { return tl ? tl[w] : w; }
 };
 
-__device__ inline RebuildWork rebuild_work(const int *plan, int nreps, const Rep *reps, int ntiles)
+// flag: the single replica's rebuild flag, passed directly so that a step
+// without a rebuild costs each kernel one load.
+__device__ inline RebuildWork rebuild_work(const int *plan, int nreps, const int *flag, int ntiles)
 {
     if (plan) return RebuildWork{plan + 4 + nreps, plan[0]};
-    return RebuildWork{nullptr, gp(reps[0].dyn)->rebuild != 0 ? ntiles : 0};
+    return RebuildWork{nullptr, *gp(flag) != 0 ? ntiles : 0};
 }
 
 // Replica of the idx-th rebuilding replica, or -1.
-__device__ inline int rebuild_rep(const int *plan, const Rep *reps, int idx)
+__device__ inline int rebuild_rep(const int *plan, const int *flag, int idx)
 {
     if (plan) return idx < plan[1] ? plan[4 + idx] : -1;
-    return (idx == 0 && gp(reps[0].dyn)->rebuild != 0) ? 0 : -1;
+    return (idx == 0 && *gp(flag) != 0) ? 0 : -1;
 }
 
 // Slot range start of cell c: chunk-local exclusive prefix + chunk offset
@@ -851,10 +853,10 @@ __device__ inline XcdRange xcd_range(int total)
 // R1: cell of every slot and its arrival rank within the cell.  Slots are
 // cell-sorted from the previous rebuild, so a wavefront spans few cells:
 // one atomic per distinct cell per wavefront.
-__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
-                                               int ntiles)
+__global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
+                                               int nreps, int ntiles)
 {
-    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
         const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
@@ -897,9 +899,9 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
 
 // R2a: chunk-local exclusive scan of the cell counts (SC_CHUNK cells per
 // block, 4 per thread); clears the counts for the next rebuild.
-__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan)
+__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan, const int *flag)
 {
-    const int r = rebuild_rep(plan, reps, (int)blockIdx.y);
+    const int r = rebuild_rep(plan, flag, (int)blockIdx.y);
     if (r < 0) return;
     const Rep R = reps[r];
     const int c0 = blockIdx.x * SC_CHUNK;
@@ -936,9 +938,9 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
 }
 
 // R2b: exclusive scan of the chunk totals (one block per replica).
-__global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *plan)
+__global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *plan, const int *flag)
 {
-    const int r = rebuild_rep(plan, reps, (int)blockIdx.x);
+    const int r = rebuild_rep(plan, flag, (int)blockIdx.x);
     if (r < 0) return;
     const Rep R = reps[r];
     const int nch = (R.ncell + SC_CHUNK - 1) / SC_CHUNK;
@@ -966,10 +968,10 @@ __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *
 
 // R3: each old slot's cell position; the cell-sorted slot gets the
 // particle's original index as its sort key.
-__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
-                                               int ntiles)
+__global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
+                                               int nreps, int ntiles)
 {
-    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
         const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
@@ -984,10 +986,10 @@ __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep 
 // is a pure function of the positions), and the live state (r/D, v, orig)
 // is scattered there.  F is dead at this point of the step (recomputed next)
 // and disp is zeroed by the build.
-__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
-                                               int ntiles)
+__global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, const int *plan,
+                                                    const int *flag, int nreps, int ntiles)
 {
-    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
         const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
@@ -1398,13 +1400,13 @@ __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long 
 // block, LPP lanes per slot; a work item is BS/LPP consecutive slots of one
 // tile.  The lists do not depend on the launch shape.
 template <int BS, int LPP>
-__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, int nreps,
-                                              int ntiles)
+__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
+                                              int nreps, int ntiles)
 {
     constexpr int SPI = BS / LPP;      // slots per work item
     constexpr int IPT = SC_TB / SPI;   // work items per tile
     static_assert(SPI * IPT == SC_TB && LPP <= WAVE, "launch shape");
-    const RebuildWork wk = rebuild_work(plan, nreps, reps, ntiles);
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     const int lane = threadIdx.x % WAVE, l = threadIdx.x % LPP;
     const unsigned long long gmask = LPP == WAVE ? ~0ull : ((1ull << LPP) - 1ull) << (lane / LPP * LPP);
     const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
@@ -1816,12 +1818,13 @@ __device__ void win_run(const Rep &R, int run, WinLds &S)
 
 // R5 (windowed form) over the planned (replica, run) pairs, or, with no plan
 // (a single replica), all of the replica's runs when its flag is set.
-__global__ __launch_bounds__(SC_TB, 3) void k_build_win(const Rep *reps, const int *plan, int nreps, int ntiles)
+__global__ __launch_bounds__(SC_TB, 3) void k_build_win(const Rep *reps, const int *plan, const int *flag, int nreps,
+                                                        int ntiles)
 {
     __shared__ WinLds S;
     const int *rl = plan ? plan + 4 + nreps + ntiles : nullptr;
     const int count = plan ? plan[2]
-                           : ((reps[0].build_kind == BUILD_WIN && gp(reps[0].dyn)->rebuild != 0) ? reps[0].nruns : 0);
+                           : ((*gp(flag) != 0 && reps[0].build_kind == BUILD_WIN) ? reps[0].nruns : 0);
     for (XcdRange it = xcd_range(count); it.cur < it.end; it.cur += it.step) {
         const int r = rl ? rl[2 * it.cur] : 0;
         const int run = rl ? rl[2 * it.cur + 1] : it.cur;
@@ -2569,12 +2572,12 @@ bool build_shape_valid(int bs, int lpp)
 
 template <int BS, int LPP>
 static void launch_build_shape(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const int *plan,
-                               hipStream_t s)
+                               const int *flag, hipStream_t s)
 {
     const int items = ntiles * (SC_TB * LPP / BS);
     const int cap = SC_GRID_CAP * (SC_TB / BS);
-    hipLaunchKernelGGL((k_build<BS, LPP>), dim3(items < cap ? items : cap), dim3(BS), 0, s, tiles, reps, plan, nreps,
-                       ntiles);
+    hipLaunchKernelGGL((k_build<BS, LPP>), dim3(items < cap ? items : cap), dim3(BS), 0, s, tiles, reps, plan, flag,
+                       nreps, ntiles);
 }
 
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
@@ -2583,34 +2586,35 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     if (ntiles <= 0) return 0;
     if (!build_shape_valid(g.build_bs, g.build_lpp)) return -1;
     // one replica: no work list, every rebuild kernel checks the flag itself
-    const int *plan = nreps == 1 ? nullptr : g.plan;
-    if (nreps > 1 && !plan) return -1;
+    const int *plan = (nreps == 1 && g.flag) ? nullptr : g.plan;
+    if (!plan && !g.flag) return -1;
     const int gt = ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP;
     if (plan) hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, ntiles, g.plan);
-    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
-    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan);
-    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan);
-    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
-    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, nreps, ntiles);
+    const int *flag = g.flag;
+    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
+    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan, flag);
+    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan, flag);
+    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
+    hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
     if (g.nruns > 0) {
         const int gr = g.nruns < SC_RUN_GRID_CAP ? g.nruns : SC_RUN_GRID_CAP;
         hipLaunchKernelGGL(k_build_cells, dim3(gr), dim3(SC_TB), 0, s, reps, nreps, g.nruns);
     }
     if (g.nwin > 0) {
         const int gw = g.nwin < SC_RUN_GRID_CAP ? g.nwin : SC_RUN_GRID_CAP;
-        hipLaunchKernelGGL(k_build_win, dim3(gw), dim3(SC_TB), 0, s, reps, plan, nreps, ntiles);
+        hipLaunchKernelGGL(k_build_win, dim3(gw), dim3(SC_TB), 0, s, reps, plan, flag, nreps, ntiles);
     }
     if (g.any_tile_build) {
         const int key = g.build_bs * 16 + g.build_lpp;
         switch (key) {
-        case 64 * 16 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, s); break;
-        case 64 * 16 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, s); break;
-        case 64 * 16 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, s); break;
-        case 64 * 16 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, s); break;
-        case 256 * 16 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, s); break;
-        case 256 * 16 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, s); break;
-        case 256 * 16 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, s); break;
-        default: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, s); break;
+        case 64 * 16 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 16 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 16 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 16 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 16 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 16 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 16 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        default: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
         }
     }
     if (bookkeep) fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
