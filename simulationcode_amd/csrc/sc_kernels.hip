@@ -196,6 +196,27 @@ __device__ inline void normals2(uint32_t orig, long long step, uint32_t sub, uin
     z1 = rr * sn;
 }
 
+// Four standard normals from one Philox block, Box-Muller in FP32 (the
+// mixed form's device noise): 24-bit uniforms u1 in (0, 1], u2 in [0, 1).
+// Counter word 3 = 2 keeps the stream apart from the FP64 form's blocks.
+__device__ inline void normals4_f32(uint32_t orig, long long step, uint64_t seed, float (&z)[4])
+{
+    uint32_t c[4] = {orig, (uint32_t)step, (uint32_t)((unsigned long long)step >> 32), 2u};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    philox4x32_10(c, k, o);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const float u1 = (float)((o[2 * q] >> 8) + 1u) * 0x1.0p-24f;
+        const float u2 = (float)(o[2 * q + 1] >> 8) * 0x1.0p-24f;
+        const float rr = sqrtf(-2.0f * logf(u1));
+        float sn, cs;
+        sincospif(2.0f * u2, &sn, &cs);
+        z[2 * q] = rr * cs;
+        z[2 * q + 1] = rr * sn;
+    }
+}
+
 // --------------------------------------------------------- pair physics
 
 // Minimum-image component: dx - round(dx/L)*L (src/AuxiliaryFunctions.jl:313).
@@ -592,6 +613,11 @@ __device__ void force_tile(const Rep &R, const Tile &t)
                                    (size_t)gp(R.orig)[i] * DIMS;
 #pragma unroll
                 for (int c = 0; c < DIMS; c++) xi[c] = nz[c];
+            } else if (PREC == 1) {
+                float z[4];
+                normals4_f32((uint32_t)gp(R.orig)[i], step, R.seed, z);
+#pragma unroll
+                for (int c = 0; c < DIMS; c++) xi[c] = (double)z[c];
             } else {
                 double z0, z1, z2, z3;
                 normals2((uint32_t)gp(R.orig)[i], step, 0u, R.seed, z0, z1);

@@ -196,6 +196,25 @@ def test_philox_noise_statistics():
     assert abs(np.mean(xi ** 4) - 3.0) < 0.15
 
 
+def test_device_noise_statistics_mixed():
+    """The mixed form draws its device noise with an FP32 Box-Muller from one
+    Philox block per particle: standard normal moments, independent axes."""
+    op, r, D, v, rc = H.make_system("berthier", 4096, 3, seed=6, system=oracle.SYS_BROWNIAN, dt=0.001)
+    ctx = make_ctx(op, rc, r, v, D, precision=nat.PREC_MIXED, seed=2024)
+    ctx.update_neighbor_lists()
+    ctx.calculate_new_forces(None)
+    F_noise = ctx.get_state(F=True)["F"]
+    st = oracle.State(op, r, v, D)
+    st.update_neighbor_lists()
+    F_pair = st.calculate_new_forces(None)
+    xi = (F_noise - F_pair) / np.sqrt(2 * op.kBT * op.gamma / op.dt)
+    assert abs(xi.mean()) < 0.02
+    assert abs(xi.std() - 1.0) < 0.02
+    assert abs(np.mean(xi ** 4) - 3.0) < 0.15
+    c = np.corrcoef(xi.T)
+    assert np.max(np.abs(c - np.eye(3))) < 0.05
+
+
 def test_overflow_raises():
     op, r, D, v, rc = H.make_system("lj", 1000, 3, seed=8, maxneighbours=40)
     ctx = make_ctx(op, rc, r, v, D)
