@@ -298,8 +298,9 @@ def main():
     ap.add_argument("--jobs", type=int, default=48, help="served: jobs per arrival rate")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the hot path would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: the MD hot path here runs on a single GPU "
+              "(replicas would shard across GPUs as independent batches, a single large system by "
+              "spatial domain decomposition; neither is built)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "copycal":
         import torch
