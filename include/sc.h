@@ -230,8 +230,8 @@ int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *dev
  * No reference counterpart: the reference's rebuild is CellListMap's CPU
  * cell list (src/NeighborLists.jl:46-52).  The per-slot neighbour-list build
  * kernel runs with block_size threads per block (64 or 256) and
- * lanes_per_particle lanes cooperating on one slot (1, 2, 4 or 8).  The
- * lists, and so every result, are identical for every shape; only the
+ * lanes_per_particle lanes cooperating on one slot (1, 2, 4, 8, 16 or 32).
+ * The lists, and so every result, are identical for every shape; only the
  * time differs (one lane per slot suits one large system, several lanes
  * suit batches of small replicas whose rebuild is latency-bound). */
 typedef struct sc_tune_entry {
@@ -240,7 +240,7 @@ typedef struct sc_tune_entry {
     double ms_per_step; /* device time per step measured for this shape */
 } sc_tune_entry;
 
-#define SC_TUNE_SHAPES 8
+#define SC_TUNE_SHAPES 12
 
 int sc_set_build_launch(sc_ctx *ctx, int32_t block_size, int32_t lanes_per_particle);
 int sc_batch_set_build_launch(sc_batch *b, int32_t block_size, int32_t lanes_per_particle);

@@ -21,6 +21,8 @@
 #define SC_TB 256  // threads per tile / block (4 wavefronts)
 #define SC_CHUNK_SHIFT 10
 #define SC_CHUNK (1 << SC_CHUNK_SHIFT)  // cells per block of the two-level scan
+#define SC_FUSED_CHUNKS 2048            // up to this many chunks the scan is one kernel
+#define SC_CHUNK_SPLIT 8                // partial counters per chunk (spreads the binning atomics)
 #define SC_WMAX 1024                    // particles of one LDS-staged build window
 #define SC_GRID_CAP 2048                // blocks of the grid-stride rebuild kernels
 #define SC_RUN_GRID_CAP 1024            // blocks of the LDS build
@@ -88,6 +90,8 @@ struct Rep {
     double *v, *v_alt, *F, *disp;
     int *orig, *orig_alt;
     int *cell_of, *rank, *cell_cnt, *cell_start, *chunk_off, *key;
+    int *chunk_cnt;      // particles per chunk of SC_CHUNK cells, SC_CHUNK_SPLIT partial counts
+                         // per chunk [chunk * SC_CHUNK_SPLIT + part] (zero between rebuilds)
     int *nn, *nbr;
     double *part;        // per-tile partials, slot-major: part[slot * ntiles + tile], 8 slots
     double *stage;       // n*3 doubles staging for host transfers
@@ -120,14 +124,15 @@ struct RebuildGeom {
     int *plan = nullptr;     // device work list, plan_ints(nreps, ntiles, nwin) ints (k_rebuild_plan)
     const int *flag = nullptr;  // one replica without a plan: its RepDyn::rebuild on the device
     int build_bs = 256;      // per-slot build launch shape: threads per block (64 or 256)
-    int build_lpp = 1;       //   and lanes per slot (1, 2, 4 or 8); the lists do not depend on it
+    int build_lpp = 1;       //   and lanes per slot (1 to 32, powers of two); the lists do not depend on it
 };
 bool build_shape_valid(int bs, int lpp);
 // rebuild work list: header {tiles, replicas, window runs, 0}, then the
 // replicas, the tiles and the (replica, run) pairs of the rebuilding replicas
 inline int plan_ints(int nreps, int ntiles, int nwin) { return 4 + nreps + ntiles + 2 * nwin; }
 // the per-slot build launch shapes tried by the autotuner, {block, lanes}
-constexpr int kBuildShapes[8][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {64, 1}, {64, 2}, {64, 4}, {64, 8}};
+constexpr int kBuildShapes[12][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {256, 16}, {256, 32},
+                                     {64, 1},  {64, 2},  {64, 4},  {64, 8},  {64, 16},  {64, 32}};
 // bookkeep=false when the next launch is a velocity-Verlet force finish,
 // which does the rebuild bookkeeping itself
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,

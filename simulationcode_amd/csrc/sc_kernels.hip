@@ -878,9 +878,12 @@ __device__ inline XcdRange xcd_range(int total)
 
 // R1: cell of every slot and its arrival rank within the cell.  Slots are
 // cell-sorted from the previous rebuild, so a wavefront spans few cells:
-// one atomic per distinct cell per wavefront.
+// one atomic per distinct cell per wavefront.  fused: the particles of each
+// chunk of SC_CHUNK cells are counted too (one atomic per distinct chunk per
+// wavefront, spread over SC_CHUNK_SPLIT partial counters by block so that
+// few wavefronts share an address), so that the scan needs no second kernel.
 __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
-                                               int nreps, int ntiles)
+                                               int nreps, int ntiles, int fused)
 {
     const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
@@ -920,12 +923,27 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
         if (group) base = gatomic_add(&gp(R.cell_cnt)[c], group);
         base = __shfl(base, leader_of, WAVE);
         if (valid) gp(R.rank)[i] = base + local;
+        if (fused) {
+            const int ch = valid ? c >> SC_CHUNK_SHIFT : -1;
+            unsigned long long act = __ballot(valid);
+            while (act) {
+                const int leader = __ffsll((long long)act) - 1;
+                const int lch = __shfl(ch, leader, WAVE);
+                const unsigned long long same = __ballot(valid && ch == lch) & act;
+                if (lane == leader)
+                    gatomic_add(&gp(R.chunk_cnt)[lch * SC_CHUNK_SPLIT + (int)(blockIdx.x % SC_CHUNK_SPLIT)],
+                                __popcll(same));
+                act &= ~same;
+            }
+        }
     }
 }
 
 // R2a: chunk-local exclusive scan of the cell counts (SC_CHUNK cells per
-// block, 4 per thread); clears the counts for the next rebuild.
-__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan, const int *flag)
+// block, 4 per thread); clears the counts for the next rebuild.  fused: the
+// block also sums the particle counts of the chunks before its own (counted
+// by k_bin) and stores its chunk's offset directly, so R2b is not launched.
+__global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan, const int *flag, int fused)
 {
     const int r = rebuild_rep(plan, flag, (int)blockIdx.y);
     if (r < 0) return;
@@ -933,6 +951,13 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
     const int c0 = blockIdx.x * SC_CHUNK;
     if (c0 >= R.ncell) return;
     const int tid = threadIdx.x;
+    if (fused) {
+        double before = 0.0;  // exact: counts are integers far below 2^53
+        for (int q = tid; q < (int)blockIdx.x * SC_CHUNK_SPLIT; q += SC_TB) before += (double)gp(R.chunk_cnt)[q];
+        double red[1] = {before};
+        block_sum<1>(red);
+        if (tid == 0) gp(R.chunk_off)[blockIdx.x] = (int)red[0];
+    }
     int v[4];
     int s = 0;
 #pragma unroll
@@ -960,7 +985,7 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
         }
         run += v[q];
     }
-    if (tid == SC_TB - 1) gp(R.chunk_off)[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
+    if (!fused && tid == SC_TB - 1) gp(R.chunk_off)[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
 }
 
 // R2b: exclusive scan of the chunk totals (one block per replica).
@@ -993,14 +1018,19 @@ __global__ __launch_bounds__(SC_TB) void k_scan_top(const Rep *reps, const int *
 }
 
 // R3: each old slot's cell position; the cell-sorted slot gets the
-// particle's original index as its sort key.
+// particle's original index as its sort key.  fused: the replica's first
+// tile clears the chunk counts that R2a has consumed.
 __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
-                                               int nreps, int ntiles)
+                                               int nreps, int ntiles, int fused)
 {
     const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
         const Tile t = tiles[wk.tile(it.cur)];
         const Rep R = reps[t.rep];
+        if (fused && t.local == 0) {
+            const int nch = (R.ncell + SC_CHUNK - 1) / SC_CHUNK;
+            for (int q = threadIdx.x; q < nch * SC_CHUNK_SPLIT; q += SC_TB) gp(R.chunk_cnt)[q] = 0;
+        }
         const int i = t.first + threadIdx.x;
         if (i >= R.n) continue;
         gp(R.key)[cstart(R, gp(R.cell_of)[i]) + gp(R.rank)[i]] = gp(R.orig)[i];
@@ -1272,21 +1302,15 @@ __device__ inline int wrap_shift(int raw, int nc, double L, double &sL)
     return raw;
 }
 
-// Candidates in the contiguous slot range [b, e) at image shift (sx, sy, sz),
-// four position loads issued before any of them is tested.
+// Candidates in the contiguous slot range [b, e) at image shift (sx, sy, sz).
+// The loop is deliberately not unrolled: the sweep is latency-bound, and
+// the registers an unrolled batch of loads holds cost more (one wavefront
+// per SIMD less) than the loads it keeps in flight.
 template <int DIMS>
 __device__ inline void build_range(const Rep &R, int k, const double4 p, int &nfull, int &cnt, int b, int e,
                                    double sx, double sy, double sz)
 {
-    int j = b;
-    for (; j + 4 <= e; j += 4) {
-        const double4 q0 = ld4(gp(R.pos_alt) + (j)), q1 = ld4(gp(R.pos_alt) + (j + 1)), q2 = ld4(gp(R.pos_alt) + (j + 2)), q3 = ld4(gp(R.pos_alt) + (j + 3));
-        if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, q0, sx, sy, sz);
-        if (j + 1 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 1, q1, sx, sy, sz);
-        if (j + 2 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 2, q2, sx, sy, sz);
-        if (j + 3 != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j + 3, q3, sx, sy, sz);
-    }
-    for (; j < e; j++)
+    for (int j = b; j < e; j++)
         if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, ld4(gp(R.pos_alt) + (j)), sx, sy, sz);
 }
 
@@ -1424,9 +1448,11 @@ __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long 
 
 // R5 (per-slot form) over the planned tiles.  Launch shape: BS threads per
 // block, LPP lanes per slot; a work item is BS/LPP consecutive slots of one
-// tile.  The lists do not depend on the launch shape.
+// tile.  The lists do not depend on the launch shape.  With one lane per
+// slot the register budget is held to five wavefronts per SIMD (96 VGPRs,
+// no spills).
 template <int BS, int LPP>
-__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
+__global__ __launch_bounds__(BS, LPP == 1 ? 5 : 1) void k_build(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
                                               int nreps, int ntiles)
 {
     constexpr int SPI = BS / LPP;      // slots per work item
@@ -2593,7 +2619,7 @@ int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s)
 
 bool build_shape_valid(int bs, int lpp)
 {
-    return (bs == 64 || bs == 256) && (lpp == 1 || lpp == 2 || lpp == 4 || lpp == 8);
+    return (bs == 64 || bs == 256) && (lpp == 1 || lpp == 2 || lpp == 4 || lpp == 8 || lpp == 16 || lpp == 32);
 }
 
 template <int BS, int LPP>
@@ -2617,10 +2643,11 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     const int gt = ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP;
     if (plan) hipLaunchKernelGGL(k_rebuild_plan, dim3(1), dim3(SC_TB), 0, s, reps, nreps, ntiles, g.plan);
     const int *flag = g.flag;
-    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
-    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan, flag);
-    hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan, flag);
-    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
+    const int fused = g.max_chunks <= SC_FUSED_CHUNKS ? 1 : 0;
+    hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles, fused);
+    hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan, flag, fused);
+    if (!fused) hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan, flag);
+    hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles, fused);
     hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
     if (g.nruns > 0) {
         const int gr = g.nruns < SC_RUN_GRID_CAP ? g.nruns : SC_RUN_GRID_CAP;
@@ -2631,16 +2658,20 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
         hipLaunchKernelGGL(k_build_win, dim3(gw), dim3(SC_TB), 0, s, reps, plan, flag, nreps, ntiles);
     }
     if (g.any_tile_build) {
-        const int key = g.build_bs * 16 + g.build_lpp;
+        const int key = g.build_bs * 64 + g.build_lpp;
         switch (key) {
-        case 64 * 16 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 16 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 16 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 16 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 16 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 16 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 16 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        default: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 16: launch_build_shape<64, 16>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 32: launch_build_shape<64, 32>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 64 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 64 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 64 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 64 + 8: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 256 * 64 + 16: launch_build_shape<256, 16>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        default: launch_build_shape<256, 32>(tiles, ntiles, reps, nreps, plan, flag, s); break;
         }
     }
     if (bookkeep) fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);

@@ -53,7 +53,7 @@ int64_t replica_bytes(const sc_params *p, const double *D)
     b += 2 * n * 32;                  // pos, pos_alt
     b += 4 * n * d * 8;               // v, v_alt, F, disp
     b += 5 * n * 4;                   // orig, orig_alt, cell_of, rank, key
-    b += (2 * ncell + 1) * 4 + 4 * ((ncell + SC_CHUNK - 1) / SC_CHUNK);
+    b += (2 * ncell + 1) * 4 + 4 * (1 + SC_CHUNK_SPLIT) * ((ncell + SC_CHUNK - 1) / SC_CHUNK);
     b += n * 4 + n * p->maxneighbours * 4;  // nn, nbr
     b += ntiles * 8 * 8 + n * 3 * 8 + 4096;  // partials, staging, dyn
     return b;
@@ -217,6 +217,7 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     rc |= A(allocs, &h.rank, n, err);
     rc |= A(allocs, &h.key, n, err);
     rc |= A(allocs, &h.chunk_off, (size_t)nchunks, err);
+    rc |= A(allocs, &h.chunk_cnt, (size_t)nchunks * SC_CHUNK_SPLIT, err);
     rc |= A(allocs, &h.cell_cnt, (size_t)ncell, err);
     rc |= A(allocs, &h.cell_start, (size_t)ncell + 1, err);
     rc |= A(allocs, &h.nn, n, err);
@@ -243,6 +244,7 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     ok &= hipMemset(h.nn, 0, sizeof(int) * n) == hipSuccess;
     ok &= hipMemset(h.part, 0, sizeof(double) * h.ntiles * 8) == hipSuccess;
     ok &= hipMemset(h.chunk_off, 0, sizeof(int) * nchunks) == hipSuccess;
+    ok &= hipMemset(h.chunk_cnt, 0, sizeof(int) * nchunks * SC_CHUNK_SPLIT) == hipSuccess;
     ok &= hipMemcpy(h.dyn, &dyn, sizeof(RepDyn), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         err = "replica initialisation copy failed";

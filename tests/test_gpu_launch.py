@@ -10,7 +10,8 @@ from simulationcode_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(256, 1), (256, 2), (256, 4), (256, 8), (64, 1), (64, 2), (64, 4), (64, 8)]
+SHAPES = [(256, 1), (256, 2), (256, 4), (256, 8), (256, 16), (256, 32),
+          (64, 1), (64, 2), (64, 4), (64, 8), (64, 16), (64, 32)]
 
 CASES = [
     dict(N=3000, dims=3, system="newtonian", dt=0.005, precision="fp64", cell_divisor=2, seed=11),
@@ -66,7 +67,7 @@ SPECS = [
 ]
 
 
-def batch_run(shape=None, tune_steps=0, steps=120):
+def batch_run(shape=None, tune_steps=0, steps=200):
     b = nat.Batch(8, graph=True)
     ids = []
     for s in SPECS:
@@ -78,7 +79,8 @@ def batch_run(shape=None, tune_steps=0, steps=120):
         table, best = b.autotune(tune_steps)
     else:
         b.set_build_launch(*shape)
-    # the tuning steps are real steps: continue to a fixed total
+    # the tuning steps (one untimed + tune_steps per shape) are real steps:
+    # continue to a fixed total
     done = b.stats(ids[0])["steps"]
     b.step(steps - done)
     out = [b.get_state(i) for i in ids]

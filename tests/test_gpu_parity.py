@@ -47,6 +47,47 @@ def test_pair_sets_bit_exact(case, cell_divisor, build_mode):
     assert ctx.rebuild_count() == 1
 
 
+def _threshold_lattice(dims, poly, jitter, seed):
+    """Simple-cubic lattice of spacing 1.5 with rc = 1.25, skin = 0.25.
+    Monodisperse (D = 1): R_v = 1.5, so the nearest-neighbour shell sits on
+    the full-list bound d2 <= R_v^2 (kept) and on the half-list bound
+    d2 < (r_c sigma + skin)^2 = R_v^2 (dropped).  poly: 10% of D = 1.2,
+    R_v = 1.75, and the D = 1 pairs of the first shell sit on their half-list
+    radius 1.5.  jitter spreads the pairs over the FP32 prefilter's error band
+    and past its edges."""
+    side = 10 if dims == 3 else 20
+    a = 1.5
+    N = side ** dims
+    L = side * a
+    rng = np.random.default_rng(seed)
+    grid = np.stack(np.meshgrid(*[np.arange(side)] * dims, indexing="ij"), -1).reshape(-1, dims) * a
+    r = grid + jitter * rng.uniform(-1.0, 1.0, grid.shape)
+    r -= np.floor(r / L) * L
+    D = np.ones(N)
+    if poly:
+        D[rng.permutation(N)[: N // 10]] = 1.2
+    op = oracle.make_params(dims=dims, N=N, box_size=L, force_cutoff=1.25, skin=0.25, D=D,
+                            potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN)
+    return op, np.ascontiguousarray(r), D
+
+
+@pytest.mark.parametrize("dims", [3, 2])
+@pytest.mark.parametrize("poly", [False, True])
+@pytest.mark.parametrize("jitter", [0.0, 1e-7, 1e-5, 3e-4])
+@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (1, 3)])
+def test_pair_sets_at_exact_thresholds(dims, poly, jitter, cell_divisor, build_mode):
+    op, r, D = _threshold_lattice(dims, poly, jitter, seed=dims * 10 + int(poly))
+    ctx = make_ctx(op, 1.25, r, np.zeros_like(r), D, cell_divisor=cell_divisor, build_mode=build_mode)
+    ctx.update_neighbor_lists()
+    nf, full, nh, half = ctx.get_neighbor_lists()
+    lst = oracle.build_lists(op, r, D, cells=False)
+    assert np.array_equal(nf, lst.nfull)
+    assert np.array_equal(oracle._pairs(nf, full - 1, both=True), lst.full_pairs())
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
+    if jitter == 0.0 and not poly:  # every first-shell pair is full-only
+        assert np.all(lst.nfull == 2 * dims) and len(lst.half_pairs()) == 0
+
+
 @pytest.mark.parametrize("case", CASES, ids=case_id)
 @pytest.mark.parametrize("system", [oracle.SYS_NEWTONIAN, oracle.SYS_LANGEVIN, oracle.SYS_BROWNIAN])
 def test_forces_fp64(case, system):
