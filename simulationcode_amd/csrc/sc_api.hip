@@ -230,6 +230,13 @@ int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, c
         for (int64_t i = 0; i < c->h.n; i++) dm = std::max(dm, (float)D[i]);
         if (dm > c->h.dmax) {
             c->h.dmax = dm;
+            // beyond zeta * max(D) = 2 the windowed build's FP32 margin no
+            // longer covers sigma_ij: continue with the exact per-slot sweep
+            if (c->h.build_kind == BUILD_WIN && c->h.pot == SC_POT_BERTHIER && c->h.zeta * (double)dm > 2.0) {
+                c->h.build_kind = BUILD_SLOT;
+                c->geom.nwin = 0;
+                c->geom.any_tile_build = 1;
+            }
             if ((rc = push_rep(c))) return rc;
         }
     }

@@ -187,7 +187,12 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     // build_mode 2) the per-slot sweep
     h.wcell = p->box_size / (double)nc;
     h.dmax = (float)maxD;
-    if (srange >= 1 && srange <= 3 && p->build_mode == 3) {
+    // the windowed build's FP32 half-list radius stays within its margin while
+    // 0 <= zeta * max(D) <= 2 (sigma_ij error <= dmax (6 zeta dmax + 4) 2^-24,
+    // threshold error <= R_v^2 (12 zeta dmax + 15) 2^-24 < 4 R_v^2 2^-20);
+    // outside that range mode 3 falls back to the exact per-slot sweep
+    const bool win_ok = p->potential != SC_POT_BERTHIER || (p->b_zeta >= 0.0 && p->b_zeta * maxD <= 2.0);
+    if (srange >= 1 && srange <= 3 && p->build_mode == 3 && win_ok) {
         h.build_kind = BUILD_WIN;
         window_geometry(h, p->dims, Rv);
     } else if (srange == 1 && p->build_mode == 1) {

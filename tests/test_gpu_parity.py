@@ -1,6 +1,6 @@
 """GPU parity: libsc (HIP, gfx950) against the oracle on identical inputs.
 
-Bars (north-star tolerances):
+Bars:
   * neighbour-pair sets: bit-exact (compared as sets of unordered pairs);
   * FP64 forces: max_i |F_gpu,i - F_oracle,i| / max_i |F_oracle,i| <= 1e-12;
   * energies: relative 1e-12;
@@ -86,6 +86,26 @@ def test_pair_sets_at_exact_thresholds(dims, poly, jitter, cell_divisor, build_m
     assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
     if jitter == 0.0 and not poly:  # every first-shell pair is full-only
         assert np.all(lst.nfull == 2 * dims) and len(lst.half_pairs()) == 0
+
+
+@pytest.mark.parametrize("build_mode", [0, 3])
+def test_pair_sets_large_zeta(build_mode):
+    """zeta * max(D) = 3.2: outside the windowed build's FP32 margin, so mode 3
+    must fall back to the exact sweep; the lists stay the oracle's."""
+    N, dims = 512, 3
+    rng = np.random.default_rng(77)
+    L = synth.box_size(N, 1.02, dims)
+    D = synth.berthier_diameters(N, rng)
+    r = synth.lattice_positions(N, dims, L, 0.1, rng)
+    op = oracle.make_params(dims=dims, N=N, box_size=L, force_cutoff=1.25, skin=0.3, D=D,
+                            potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN, zeta=2.0)
+    assert 2.0 * D.max() > 2.0
+    ctx = make_ctx(op, 1.25, r, np.zeros_like(r), D, cell_divisor=2, build_mode=build_mode)
+    ctx.update_neighbor_lists()
+    nf, full, nh, half = ctx.get_neighbor_lists()
+    lst = oracle.build_lists(op, r, D, cells=False)
+    assert np.array_equal(nf, lst.nfull)
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
 
 
 @pytest.mark.parametrize("case", CASES, ids=case_id)
