@@ -104,26 +104,6 @@ def drift_run(ctx, steps, samples=20):
     return float(slope * 1e5 / abs(e[0])), float((max(e) - min(e)) / abs(e[0]))
 
 
-def copy_ceiling_gbs(nbytes=1 << 30, reps=10):
-    """Measured HBM ceiling: a plain device-to-device copy of `nbytes`
-    (read + write counted), timed with events on torch's stream."""
-    import torch
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-    b = torch.empty_like(a)
-    b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    del a, b
-    torch.cuda.empty_cache()
-    return 2.0 * nbytes * reps / (ms / 1e3) / 1e9
-
-
 def cpu_baseline(op_state, budget_s=15.0):
     import oracle
     r, v, F, D, L, N = op_state
@@ -154,6 +134,7 @@ def bench_large(args, nat, synth):
     r = synth.lattice_positions(N, dims, L, 0.1, rng)
 
     log(f"[bench] setup N={N} L={L:.3f}")
+    cc = None if args.no_copy_ceiling else nat.copy_bandwidth()  # streaming ceiling, 1 GiB copies
     ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64, cell_divisor=args.cell_divisor)
     ctx.set_state(r=r, v=np.zeros((N, dims)), F=np.zeros((N, dims)))
     ctx.update_neighbor_lists()
@@ -188,9 +169,9 @@ def bench_large(args, nat, synth):
                 "kernel_avg_us": round(f_avg_s * 1e6, 2), "bytes_per_particle": round(bytes_per_particle, 2),
                 "n_half_mean": round(n_half, 3),
                 "drift_kernel_avg_us": round(kt["drift_ms"] / max(kt["drift_launches"], 1) * 1e3, 2)}
-    cc = copy_ceiling_gbs()
-    roofline.update({"copy_ceiling_GBs": round(cc, 1), "frac_of_copy_ceiling": round(achieved / cc, 4)})
-    log(f"[bench] measured copy ceiling {cc:.0f} GB/s")
+    if cc:
+        roofline.update({"copy_ceiling_GBs": round(cc, 1), "frac_of_copy_ceiling": round(achieved / cc, 4)})
+        log(f"[bench] measured copy ceiling {cc:.0f} GB/s")
     out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64",
@@ -318,6 +299,7 @@ def main():
                     help="copycal: five plain 256 MiB device copies, the reference reading for HBM counters")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-copy-ceiling", action="store_true", help="large: skip the measured device-copy ceiling")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--drift-steps", type=int, default=4000)
     ap.add_argument("--replicas", type=int, default=256)

@@ -161,6 +161,10 @@ void sc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
 /* Device-side wall time of the last sc_do_time_steps call (HIP events on the
  * context stream), in milliseconds. */
 int sc_last_elapsed_ms(sc_ctx *ctx, double *ms);
+/* Measurement hook, no reference counterpart: the device's streaming
+ * ceiling, a plain copy of `bytes` (read + write counted) timed over `reps`
+ * launches; allocates and frees its own buffers. */
+int sc_copy_bandwidth(int64_t bytes, int32_t reps, double *gbps);
 
 /* Measurement hooks (no reference counterpart; the reference only wraps its
  * loop in @time, src/MD.jl:704).  With timing on, sc_do_time_steps brackets

@@ -44,7 +44,7 @@ EXPORTS = (
     "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_calculate_new_forces",
     "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
     "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_philox4x32_10",
-    "sc_last_elapsed_ms", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
+    "sc_last_elapsed_ms", "sc_copy_bandwidth", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
     "sc_batch_create", "sc_batch_destroy", "sc_batch_last_error", "sc_replica_device_bytes",
     "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
@@ -154,6 +154,7 @@ def load(path=LIB_PATH):
         "sc_minimize": (ctypes.c_int, [vp, i64, dbl, dbl, pd]),
         "sc_philox4x32_10": (None, [vp, vp, vp]),
         "sc_last_elapsed_ms": (ctypes.c_int, [vp, pd]),
+        "sc_copy_bandwidth": (ctypes.c_int, [i64, ctypes.c_int32, pd]),
         "sc_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "sc_kernel_times": (ctypes.c_int, [vp, pd, ctypes.POINTER(i64), pd, ctypes.POINTER(i64)]),
         "sc_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
@@ -197,6 +198,16 @@ def _p(a):
     if not (a.flags["C_CONTIGUOUS"] and a.dtype == np.float64):
         raise TypeError("arrays crossing the C-ABI must be C-contiguous float64")
     return a.ctypes.data
+
+
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """GB/s of a plain device copy (read + write counted): the measured
+    streaming ceiling for the HBM-bound kernels."""
+    g = ctypes.c_double()
+    rc = load().sc_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(g))
+    if rc != 0:
+        raise ScError(rc, "sc_copy_bandwidth failed")
+    return g.value
 
 
 def philox4x32_10(ctr, key):

@@ -528,6 +528,35 @@ int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *ma
     return check_dyn(c);
 }
 
+int sc_copy_bandwidth(int64_t bytes, int32_t reps, double *gbps)
+{
+    if (bytes < 1024 || reps < 1 || !gbps) return SC_ERR_ARG;
+    const size_t n = (size_t)bytes / sizeof(double4);
+    double4 *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SC_OK;
+    if (hipMalloc(&a, n * sizeof(double4)) != hipSuccess || hipMalloc(&b, n * sizeof(double4)) != hipSuccess)
+        rc = SC_ERR_OOM;
+    if (rc == SC_OK && (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+                        hipEventCreate(&e1) != hipSuccess || hipMemsetAsync(a, 0, n * sizeof(double4), s) != hipSuccess))
+        rc = SC_ERR_HIP;
+    if (rc == SC_OK) rc = sc::launch_copy(a, b, n, s) ? SC_ERR_HIP : SC_OK;  // warm-up
+    if (rc == SC_OK && hipEventRecord(e0, s) != hipSuccess) rc = SC_ERR_HIP;
+    for (int r = 0; rc == SC_OK && r < reps; r++) rc = sc::launch_copy(a, b, n, s) ? SC_ERR_HIP : SC_OK;
+    float ms = 0.0f;
+    if (rc == SC_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.0f)))
+        rc = SC_ERR_HIP;
+    if (rc == SC_OK) *gbps = 2.0 * (double)(n * sizeof(double4)) * reps / ((double)ms * 1e-3) / 1e9;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int sc_last_elapsed_ms(sc_ctx *c, double *ms)
 {
     if (!c || !ms) return SC_ERR_ARG;

@@ -109,6 +109,7 @@ struct Tile {
 // one-block-per-replica k_finish that reduces its tile partials
 namespace sc {
 int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_copy(const double4 *src, double4 *dst, size_t n, hipStream_t s);
 int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s,
                  bool finish = true);
 int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s);

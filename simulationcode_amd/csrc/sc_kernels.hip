@@ -2546,6 +2546,20 @@ static inline void fin(const Rep *reps, int nreps, int op, int mask, double arg,
     hipLaunchKernelGGL(k_finish, dim3(nreps), dim3(SC_TB), 0, s, reps, nreps, op, mask, arg);
 }
 
+// Plain streaming copy (the measured HBM ceiling of the measurement hook).
+__global__ __launch_bounds__(SC_TB) void k_copy(const double4 *src, double4 *dst, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)SC_TB + threadIdx.x; i < n; i += (size_t)gridDim.x * SC_TB)
+        st4(gp(dst) + i, ld4(gp(src) + i));
+}
+
+int launch_copy(const double4 *src, double4 *dst, size_t n, hipStream_t s)
+{
+    const size_t blocks = (n + SC_TB - 1) / SC_TB;
+    hipLaunchKernelGGL(k_copy, dim3(blocks < 8192 ? (unsigned)blocks : 8192u), dim3(SC_TB), 0, s, src, dst, n);
+    return check(hipGetLastError());
+}
+
 int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
     if (ntiles <= 0) return 0;

@@ -116,3 +116,12 @@ def test_context_autotune():
     st = ctx.get_state()
     assert np.all(np.isfinite(st["r"]))
     ctx.close()
+
+
+def test_copy_bandwidth_hook():
+    """The streaming-copy measurement hook: a plausible HBM figure, bad
+    arguments rejected."""
+    g = nat.copy_bandwidth(1 << 28, 5)
+    assert 500.0 < g < 9000.0
+    with pytest.raises(nat.ScError):
+        nat.copy_bandwidth(16, 1)

@@ -88,6 +88,27 @@ def test_pair_sets_at_exact_thresholds(dims, poly, jitter, cell_divisor, build_m
         assert np.all(lst.nfull == 2 * dims) and len(lst.half_pairs()) == 0
 
 
+def test_pair_sets_two_kernel_scan():
+    """N=5e5 at cell divisor 4: 136^3 cells = 2457 chunks of 1024, above the
+    2048 up to which the cell scan is one kernel, so the chunk totals go
+    through the separate top-level scan."""
+    N, dims = 500000, 3
+    rng = np.random.default_rng(3)
+    L = synth.box_size(N, 1.02, dims)
+    D = synth.berthier_diameters(N, rng)
+    r = synth.lattice_positions(N, dims, L, 0.1, rng)
+    op = oracle.make_params(dims=dims, N=N, box_size=L, force_cutoff=1.25, skin=0.3, D=D,
+                            potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN)
+    nc = int(np.floor(4 * L / (op.verlet_sphere_size * (1 + 1e-10))))
+    assert (nc ** 3 + 1023) // 1024 > 2048
+    ctx = make_ctx(op, 1.25, r, np.zeros_like(r), D, cell_divisor=4)
+    ctx.update_neighbor_lists()
+    nf, full, nh, half = ctx.get_neighbor_lists(96)  # rows hold <= 65 here
+    lst = oracle.build_lists(op, r, D, cells=True)
+    assert np.array_equal(nf, lst.nfull)
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
+
+
 @pytest.mark.parametrize("build_mode", [0, 3])
 def test_pair_sets_large_zeta(build_mode):
     """zeta * max(D) = 3.2: outside the windowed build's FP32 margin, so mode 3
