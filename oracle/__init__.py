@@ -108,6 +108,8 @@ def lib():
         for f in (L.or_ql, L.or_ql_averaged):
             f.argtypes = [P, dp, dp, dp, ctypes.c_int, ctypes.c_double]
             f.restype = ctypes.c_double
+        L.or_baseline_steps.restype = ctypes.c_int
+        L.or_baseline_steps.argtypes = [P, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -219,6 +221,21 @@ def kinetic_energy(p, v):
 
 def rescale_velocities(p, v):
     lib().or_rescale_velocities(ctypes.byref(p), _ptr(v))
+
+
+def baseline_steps(p, nsteps, nthreads, r, v, F, D):
+    """All-core CPU timing baseline (not a checker): nsteps Newtonian
+    velocity-Verlet steps on nthreads OpenMP threads, lists rebuilt on entry.
+    r, v, F (N x dims FP64) are advanced in place; returns the rebuilds."""
+    for a in (r, v, F):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    D = np.ascontiguousarray(D, dtype=np.float64)
+    rebuilds = np.zeros(1, dtype=np.int64)
+    rc = lib().or_baseline_steps(ctypes.byref(p), int(nsteps), int(nthreads), _ptr(r), _ptr(v), _ptr(F),
+                                 _ptr(D), _ptr(rebuilds))
+    if rc != 0:
+        raise RuntimeError(f"baseline steps failed: {rc}")
+    return int(rebuilds[0])
 
 
 class State:

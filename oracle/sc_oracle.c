@@ -2,7 +2,8 @@
  * sc_oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain-C, FP64, single-threaded restatement of SimulationCode.jl's
- * molecular-dynamics hot path.  It is the checker for the HIP library and the
+ * molecular-dynamics hot path (plus, at the end, an all-core OpenMP form of
+ * its Newtonian step that serves only as a timing baseline).  It is the checker for the HIP library and the
  * CPU baseline ("port") of the benchmark.  Nothing in the product links it:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.
@@ -27,6 +28,7 @@
  *     CellListMap internals; compare lists as sets.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -691,4 +693,190 @@ double or_ql_averaged(const or_params *p, const double *r, const int64_t *nfull,
     free(qim);
     free(nqa);
     return avg / (double)N;
+}
+
+/* ------------------------------------------------------------------------
+ * All-core CPU baseline (timing only; not a checker)
+ *
+ * The Newtonian velocity-Verlet step of or_do_time_steps with its loops over
+ * particles split across OpenMP threads, the parallel form SimulationCode.jl
+ * would take with threads: a serial linked-cell binning (cells of side >=
+ * R_v), a parallel per-particle stencil sweep in which every particle writes
+ * only its own row of partners with d2 < (r_c sigma_ij + skin)^2 (both
+ * directions, so no thread writes another particle's row), and forces
+ * gathered over those rows (each pair evaluated from both sides, no scatter).
+ * Sums run in another order than the serial oracle's, so trajectories agree
+ * to rounding, not bit for bit.  Full-list rows are counted for the
+ * maxneighbours check but not stored.
+ * ------------------------------------------------------------------------ */
+
+static int baseline_rebuild(const or_params *p, const double *r, const double *D, int32_t *nshort,
+                            int32_t *shortl, double *disp)
+{
+    int64_t N = p->N, M = p->maxneighbours;
+    int dims = p->dims;
+    double L = p->box_size, Rv = p->verlet_sphere_size, Rv2 = Rv * Rv;
+    double r_cutoff = sqrt(p->force_cutoff2);
+    int64_t nc = (int64_t)floor(L / Rv);
+    if (nc < 3) nc = 1;
+    double cs = L / (double)nc;
+    int64_t ncell = nc * nc * (dims == 3 ? nc : 1);
+    int64_t *head = (int64_t *)malloc(sizeof(int64_t) * ncell);
+    int64_t *next = (int64_t *)malloc(sizeof(int64_t) * N);
+    int64_t *cell = (int64_t *)malloc(sizeof(int64_t) * N);
+    if (!head || !next || !cell) { free(head); free(next); free(cell); return OR_ERR_ARG; }
+    for (int64_t c = 0; c < ncell; c++) head[c] = -1;
+    for (int64_t i = N - 1; i >= 0; i--) {
+        int64_t ci[3] = {0, 0, 0};
+        for (int c = 0; c < dims; c++) {
+            int64_t k = (int64_t)floor(r[i * dims + c] / cs);
+            if (k < 0) k = 0;
+            if (k >= nc) k = nc - 1;
+            ci[c] = k;
+        }
+        int64_t id = ci[0] + nc * (ci[1] + nc * ci[2]);
+        cell[i] = id;
+        next[i] = head[id];
+        head[id] = i;
+    }
+    int overflow = 0;
+#pragma omp parallel for schedule(static) reduction(| : overflow)
+    for (int64_t i = 0; i < N; i++) {
+        int64_t id = cell[i];
+        int64_t cx = id % nc, cy = (id / nc) % nc, cz = id / (nc * nc);
+        int sr = nc == 1 ? 0 : 1;
+        int zr = (dims == 3) ? sr : 0;
+        int32_t ns = 0, nf = 0;
+        for (int dz = -zr; dz <= zr; dz++)
+            for (int dy = -sr; dy <= sr; dy++)
+                for (int dx = -sr; dx <= sr; dx++) {
+                    int64_t nx = (cx + dx + nc) % nc, ny = (cy + dy + nc) % nc;
+                    int64_t nz = (dims == 3) ? (cz + dz + nc) % nc : 0;
+                    for (int64_t j = head[nx + nc * (ny + nc * nz)]; j >= 0; j = next[j]) {
+                        if (j == i) continue;
+                        double d2 = or_squared_distance(r + i * dims, r + j * dims, dims, L);
+                        if (d2 > Rv2) continue;
+                        nf++;
+                        double md = mean_d(p, D[i], D[j]);
+                        double rh = r_cutoff * md + p->skin_distance;
+                        if (d2 < rh * rh) {
+                            if (ns < M) shortl[i * M + ns] = (int32_t)j;
+                            ns++;
+                        }
+                    }
+                }
+        if (nf > M) overflow = 1;
+        nshort[i] = ns < M ? ns : (int32_t)M;
+        for (int c = 0; c < dims; c++) disp[i * dims + c] = 0.0;
+    }
+    free(head); free(next); free(cell);
+    return overflow ? OR_ERR_OVERFLOW : OR_OK;
+}
+
+static void baseline_forces(const or_params *p, const double *r, const double *D, const int32_t *nshort,
+                            const int32_t *shortl, double *F)
+{
+    int64_t N = p->N, M = p->maxneighbours;
+    int dims = p->dims;
+    double L = p->box_size, fc2 = p->force_cutoff2;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; i++) {
+        const double *ri = r + i * dims;
+        double Di = D[i];
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int32_t k = 0; k < nshort[i]; k++) {
+            int64_t j = shortl[i * M + k];
+            const double *rj = r + j * dims;
+            double dr[3] = {0.0, 0.0, 0.0};
+            for (int c = 0; c < dims; c++) {
+                dr[c] = ri[c] - rj[c];
+                dr[c] -= rint(dr[c] / L) * L;
+            }
+            double rij2 = dr[0] * dr[0] + dr[1] * dr[1];
+            if (dims == 3) rij2 = rij2 + dr[2] * dr[2];
+            double Fs;
+            if (p->potential == OR_POT_BERTHIER) {
+                double md = (Di + D[j]) * 0.5 * (1.0 - p->b_zeta * fabs(Di - D[j]));
+                double md2 = md * md;
+                double inv_md2 = 1.0 / md2;
+                double invxi2 = rij2 * inv_md2;
+                double invxi4 = invxi2 * invxi2;
+                double xi14 = 1.0 / (invxi4 * invxi4 * invxi4 * invxi2);
+                Fs = (fc2 * md2 < rij2) ? 0.0 : -2.0 * inv_md2 * (p->b_c2 + 2.0 * p->b_c4 * invxi2 - 6.0 * xi14);
+            } else if (dims == 3) {
+                double s2 = p->lj_sigma * p->lj_sigma;
+                double xi2 = s2 / rij2;
+                double xi6 = xi2 * xi2 * xi2;
+                double xi12 = xi6 * xi6;
+                Fs = (fc2 < rij2) ? 0.0 : p->lj_eps * (48.0 * xi12 - 24.0 * xi6) / rij2;
+            } else {
+                double md = mean_d(p, Di, D[j]);
+                double md2 = md * md;
+                Fs = (fc2 * md2 > rij2) ? force_divr(p, rij2, md2) : 0.0;
+            }
+            for (int c = 0; c < dims; c++) f[c] += Fs * dr[c];
+        }
+        for (int c = 0; c < dims; c++) F[i * dims + c] = f[c];
+    }
+}
+
+/* nsteps Newtonian velocity-Verlet steps (src/MD.jl:499-541 operation by
+ * operation) on `nthreads` threads.  The lists are rebuilt on entry. */
+int or_baseline_steps(const or_params *p, int64_t nsteps, int nthreads, double *r, double *v, double *F,
+                      const double *D, int64_t *rebuilds)
+{
+    if (p->system != OR_SYS_NEWTONIAN || nthreads < 1) return OR_ERR_ARG;
+    int64_t N = p->N, M = p->maxneighbours;
+    int dims = p->dims;
+    double dt = p->dt, m = p->m, L = p->box_size;
+    double thr = (p->skin_distance / 2.0) * (p->skin_distance / 2.0);
+    int32_t *nshort = (int32_t *)malloc(sizeof(int32_t) * N);
+    int32_t *shortl = (int32_t *)malloc(sizeof(int32_t) * N * M);
+    double *disp = (double *)malloc(sizeof(double) * N * dims);
+    if (!nshort || !shortl || !disp) { free(nshort); free(shortl); free(disp); return OR_ERR_ARG; }
+    int prev = omp_get_max_threads();
+    omp_set_num_threads(nthreads);
+    int rc = baseline_rebuild(p, r, D, nshort, shortl, disp);
+    for (int64_t s = 0; s < nsteps && rc == OR_OK; s++) {
+        double mx = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+        for (int64_t i = 0; i < N; i++) {
+            double s2 = 0.0;
+            for (int c = 0; c < dims; c++) {
+                int64_t q = i * dims + c;
+                v[q] = v[q] + dt * F[q] / (2.0 * m);
+                double dr = dt * v[q];
+                disp[q] += dr;
+                s2 = (c == 0) ? disp[q] * disp[q] : s2 + disp[q] * disp[q];
+                double rn = r[q] + dr;
+                rn -= floor(rn / L) * L;
+                r[q] = rn;
+            }
+            if (s2 > mx) mx = s2;
+        }
+        if (mx > thr) {
+            (*rebuilds)++;
+            rc = baseline_rebuild(p, r, D, nshort, shortl, disp);
+            if (rc) break;
+        }
+        baseline_forces(p, r, D, nshort, shortl, F);
+        double vm0 = 0.0, vm1 = 0.0, vm2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : vm0, vm1, vm2)
+        for (int64_t i = 0; i < N; i++) {
+            for (int c = 0; c < dims; c++) {
+                int64_t q = i * dims + c;
+                v[q] = v[q] + dt * F[q] / (2.0 * m);
+            }
+            vm0 += v[i * dims];
+            vm1 += v[i * dims + 1];
+            if (dims == 3) vm2 += v[i * dims + 2];
+        }
+        double vm[3] = {vm0 / (double)N, vm1 / (double)N, vm2 / (double)N};
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < N; i++)
+            for (int c = 0; c < dims; c++) v[i * dims + c] -= vm[c];
+    }
+    omp_set_num_threads(prev);
+    free(nshort); free(shortl); free(disp);
+    return rc;
 }

@@ -294,3 +294,24 @@ def test_planted_mistake_is_caught():
     iu, ju = np.triu_indices(op.N, 1)
     bad = np.sort((iu.astype(np.int64) * op.N + ju)[d2[iu, ju] <= op.verlet_sphere_size ** 2])
     assert not np.array_equal(bad, full_np)
+
+
+@pytest.mark.parametrize("kind,dims,N", [("berthier", 3, 512), ("lj", 3, 343), ("berthier", 2, 400)])
+def test_all_core_baseline_follows_the_serial_oracle(kind, dims, N):
+    """The OpenMP timing baseline (per-particle rows, gathered forces) runs
+    the same dynamics as the serial oracle: trajectories agree to rounding
+    over 60 steps with the same rebuild count, on 1 and on 3 threads."""
+    op, r, D, v, rc = H.make_system(kind, N, dims, seed=900 + N, kBT=0.3, dt=0.005)
+    st = oracle.State(op, r, v, D)
+    st.update_neighbor_lists()
+    st.calculate_new_forces(None)
+    F0 = st.F.copy()
+    st.do_time_steps(60)
+    for threads in (1, 3):
+        rb, vb, Fb = r.copy(), v.copy(), F0.copy()
+        nreb = oracle.baseline_steps(op, 60, threads, rb, vb, Fb, D)
+        dr = rb - st.r
+        dr -= np.round(dr / op.box_size) * op.box_size
+        assert np.max(np.abs(dr)) <= 1e-9
+        assert np.max(np.abs(vb - st.v)) <= 1e-9
+        assert nreb == st.rebuilds[0] - 1
