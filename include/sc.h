@@ -116,6 +116,10 @@ int sc_do_time_steps(sc_ctx *ctx, int64_t nsteps, const double *noise);
 /* calculate_full_energy -- src/AuxiliaryFunctions.jl:22-34 (+ find_energy
  * :546-584): sum over full-list pairs, halved; compensated reduction. */
 int sc_full_energy(sc_ctx *ctx, double *out);
+/* compute_potential_energy! -- src/AuxiliaryFunctions.jl:186-196: E[i] =
+ * find_energy(i) / 2 for every particle (original order, N doubles); the
+ * potential_energy_array that save_data dumps (src/Dump.jl:82-85). */
+int sc_potential_energies(sc_ctx *ctx, double *E);
 /* calculate_full_energy_no_neigh -- src/AuxiliaryFunctions.jl:86-110: the
  * O(N^2) all-pairs energy, tiled on the device. */
 int sc_energy_no_neigh(sc_ctx *ctx, double *out);

@@ -108,6 +108,7 @@ def lib():
         for f in (L.or_ql, L.or_ql_averaged):
             f.argtypes = [P, dp, dp, dp, ctypes.c_int, ctypes.c_double]
             f.restype = ctypes.c_double
+        L.or_potential_energies.argtypes = [P, dp, dp, dp, dp, dp]
         L.or_baseline_steps.restype = ctypes.c_int
         L.or_baseline_steps.argtypes = [P, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         _lib = L
@@ -205,6 +206,13 @@ def calculate_new_forces(p, r, D, lists, v=None, noise=None):
     lib().or_calculate_new_forces(ctypes.byref(p), _ptr(r), _ptr(D), _ptr(lists.nhalf),
                                   _ptr(lists.half), _ptr(v), _ptr(noise), _ptr(F))
     return F
+
+
+def potential_energies(p, r, D, lists):
+    """compute_potential_energy!: E_i / 2 per particle."""
+    E = np.zeros(p.N)
+    lib().or_potential_energies(ctypes.byref(p), _ptr(r), _ptr(D), _ptr(lists.nfull), _ptr(lists.full), _ptr(E))
+    return E
 
 
 def full_energy(p, r, D, lists):

@@ -384,6 +384,14 @@ static double find_energy(const or_params *p, int64_t i, const double *r, const 
 }
 
 /* src/AuxiliaryFunctions.jl:22-34 */
+/* src/AuxiliaryFunctions.jl:186-196 (compute_potential_energy!): E[i] =
+ * find_energy(i) / 2. */
+void or_potential_energies(const or_params *p, const double *r, const double *D, const int64_t *nfull,
+                           const int64_t *full, double *E)
+{
+    for (int64_t i = 0; i < p->N; i++) E[i] = find_energy(p, i, r, D, nfull, full) / 2.0;
+}
+
 double or_full_energy(const or_params *p, const double *r, const double *D,
                       const int64_t *nfull, const int64_t *full)
 {

@@ -12,5 +12,6 @@ from .md import (  # noqa: F401
     NeighborListStruct, initialize_neighbor_struct, update_neighbor_lists, calculate_new_forces,
     do_time_step, rescale_velocities, calculate_full_energy, calculate_full_energy_no_neigh,
     find_kinetic_energy, logdata, create_when_to_save_array, run_short_MD_equilibration,
-    perform_molecular_dynamics, find_inherent_structure, save_data, seed,
+    perform_molecular_dynamics, find_inherent_structure, save_data, seed, compute_potential_energy,
+    prepare_savefile, dump_writer, read_restart,
 )

@@ -44,7 +44,7 @@ EXPORTS = (
     "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_calculate_new_forces",
     "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
     "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_philox4x32_10",
-    "sc_last_elapsed_ms", "sc_copy_bandwidth", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
+    "sc_last_elapsed_ms", "sc_copy_bandwidth", "sc_potential_energies", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
     "sc_batch_create", "sc_batch_destroy", "sc_batch_last_error", "sc_replica_device_bytes",
     "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
@@ -155,6 +155,7 @@ def load(path=LIB_PATH):
         "sc_philox4x32_10": (None, [vp, vp, vp]),
         "sc_last_elapsed_ms": (ctypes.c_int, [vp, pd]),
         "sc_copy_bandwidth": (ctypes.c_int, [i64, ctypes.c_int32, pd]),
+        "sc_potential_energies": (ctypes.c_int, [vp, vp]),
         "sc_set_kernel_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "sc_kernel_times": (ctypes.c_int, [vp, pd, ctypes.POINTER(i64), pd, ctypes.POINTER(i64)]),
         "sc_neighbor_stats": (ctypes.c_int, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
@@ -317,6 +318,12 @@ class Context:
 
     def full_energy(self):
         return self._scalar(self._lib.sc_full_energy)
+
+    def potential_energies(self):
+        """compute_potential_energy!: E_i / 2 per particle, original order."""
+        E = np.zeros(self.N)
+        self._check(self._lib.sc_potential_energies(self._h, E.ctypes.data))
+        return E
 
     def energy_no_neigh(self):
         return self._scalar(self._lib.sc_energy_no_neigh)

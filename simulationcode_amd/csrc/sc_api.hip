@@ -410,6 +410,19 @@ int sc_full_energy(sc_ctx *c, double *out)
     return SC_OK;
 }
 
+int sc_potential_energies(sc_ctx *c, double *E)
+{
+    if (!c || !c->d_rep || !E) return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    KCHK(c, sc::launch_energy(c->d_tiles, c->ntiles, c->d_rep, 1, c->s, c->h.stage));
+    HIPCHK(c, hipMemcpyAsync(E, c->h.stage, sizeof(double) * c->h.n, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    return SC_OK;
+}
+
 int sc_energy_no_neigh(sc_ctx *c, double *out)
 {
     if (!c || !c->d_rep || !out) return SC_ERR_ARG;

@@ -33,6 +33,8 @@
 // neighbour-list build kinds of a replica
 enum { BUILD_SLOT = 0, BUILD_CELLS = 1, BUILD_WIN = 2 };
 
+#define REBUILD_KEEP_F 2
+
 // error bits raised on the device
 #define SC_EBIT_OVERFLOW 1
 #define SC_EBIT_NONFINITE 2
@@ -43,7 +45,8 @@ enum { CTR_FORCE = 0, CTR_DRIFT, CTR_BUILD, CTR_ENERGY, CTR_EKIN, CTR_MISC, CTR_
 struct RepDyn {
     unsigned long long maxdisp_bits;  // atomicMax over disp^2 (non-negative doubles)
     unsigned long long maxaux_bits;   // atomicMax scratch (minimiser step / force)
-    int rebuild;                      // 1 => the rebuild kernels run
+    int rebuild;                      // nonzero => the rebuild kernels run; bit 1 (REBUILD_KEEP_F):
+                                      // requested between steps, the stored F is live and moves along
     int err;                          // SC_EBIT_*
     int pending;                      // 1 => vmean / vscale not yet applied to v
     int have_lists;
@@ -141,7 +144,8 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
 int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s);
 int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
-int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+// per (optional): E_i / 2 of every particle at its original index
+int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, double *per = nullptr);
 int launch_energy_nn(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_ekin(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int apply_rescale, hipStream_t s);
 int launch_set_rold(const Tile *tiles, int ntiles, const Rep *reps, double *rold, hipStream_t s);

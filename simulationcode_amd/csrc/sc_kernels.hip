@@ -736,7 +736,7 @@ __global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep
 __global__ void k_request_rebuild(Rep *reps, int nreps)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nreps) gp(reps[r].dyn)->rebuild = 1;
+    if (r < nreps) gp(reps[r].dyn)->rebuild = 1 | REBUILD_KEEP_F;  // between steps: F stays live
 }
 
 // Error bits of every replica into one array (one host copy per check).
@@ -1040,8 +1040,9 @@ __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep 
 // R4: final slot = cell start + rank of the original index among the cell's
 // members (so the order within a cell is by original index: the slot order
 // is a pure function of the positions), and the live state (r/D, v, orig)
-// is scattered there.  F is dead at this point of the step (recomputed next)
-// and disp is zeroed by the build.
+// is scattered there.  Inside a step F is dead at this point (recomputed
+// next); a rebuild requested between steps (REBUILD_KEEP_F) moves F too,
+// through the staging buffer.  disp is zeroed by the build.
 __global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const Rep *reps, const int *plan,
                                                     const int *flag, int nreps, int ntiles)
 {
@@ -1059,6 +1060,8 @@ __global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const
         const int k = s0 + rank;
         st4(gp(R.pos_alt) + (k), ld4(gp(R.pos) + (x)));
         for (int d = 0; d < R.dims; d++) gp(R.v_alt)[d * n + k] = gp(R.v)[d * n + x];
+        if (gp(R.dyn)->rebuild & REBUILD_KEEP_F)
+            for (int d = 0; d < R.dims; d++) gp(R.stage)[d * n + k] = gp(R.F)[d * n + x];
         gp(R.orig_alt)[k] = oid;
     }
 }
@@ -1097,9 +1100,11 @@ __device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, i
     if (nfull > R.cap) gatomic_or(&gp(R.dyn)->err, SC_EBIT_OVERFLOW);
     gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
     st4(gp(R.pos) + (k), p);
+    const bool keepF = (gp(R.dyn)->rebuild & REBUILD_KEEP_F) != 0;
     for (int c = 0; c < DIMS; c++) {
         gp(R.v)[c * n + k] = gp(R.v_alt)[c * n + k];
         gp(R.disp)[c * n + k] = 0.0;
+        if (keepF) gp(R.F)[c * n + k] = gp(R.stage)[c * n + k];
     }
     gp(R.orig)[k] = gp(R.orig_alt)[k];
 }
@@ -1936,9 +1941,11 @@ __global__ __launch_bounds__(SC_TB) void k_full_rows(const Tile *tiles, const Re
 // calculate_full_energy (src/AuxiliaryFunctions.jl:22-34): per-particle sums
 // over every candidate of the rebuild-time cell stencil with the inner loop's
 // cutoff test, which selects exactly the full-list pairs inside the cutoff;
-// Neumaier-compensated tile and replica reductions; total halved.
+// Neumaier-compensated tile and replica reductions; total halved.  per: also
+// compute_potential_energy! (src/AuxiliaryFunctions.jl:186-196), E_i / 2 of
+// every particle at its original index.
 template <int DIMS, int POT>
-__device__ void energy_tile(const Rep &R, const Tile &t)
+__device__ void energy_tile(const Rep &R, const Tile &t, double *per)
 {
     SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int n = R.n, k = t.first + threadIdx.x;
@@ -1962,6 +1969,7 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
                         e += pair_energy<DIMS, POT>(R, d2, p.w, q.w);
                     }
                 }
+        if (per) gp(per)[gp(R.orig)[k]] = e / 2.0;
     }
     // compensated block reduction: exact two-sum tree is overkill here; a
     // per-wave Neumaier pass over lane values keeps the error at O(eps).
@@ -1976,16 +1984,16 @@ __device__ void energy_tile(const Rep &R, const Tile &t)
     }
 }
 
-__global__ __launch_bounds__(SC_TB) void k_energy(const Tile *tiles, const Rep *reps)
+__global__ __launch_bounds__(SC_TB) void k_energy(const Tile *tiles, const Rep *reps, double *per)
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     const int key = (R.dims == 3 ? 2 : 0) + (R.pot == 2 ? 1 : 0);
     switch (key) {
-    case 0: energy_tile<2, 1>(R, t); break;
-    case 1: energy_tile<2, 2>(R, t); break;
-    case 2: energy_tile<3, 1>(R, t); break;
-    default: energy_tile<3, 2>(R, t); break;
+    case 0: energy_tile<2, 1>(R, t, per); break;
+    case 1: energy_tile<2, 2>(R, t, per); break;
+    case 2: energy_tile<3, 1>(R, t, per); break;
+    default: energy_tile<3, 2>(R, t, per); break;
     }
 }
 
@@ -2700,10 +2708,10 @@ int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, h
     return check(hipGetLastError());
 }
 
-int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, double *per)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_energy, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_energy, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, per);
     fin(reps, nreps, FIN_ENERGY, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }

@@ -26,10 +26,16 @@ Reference map:
   calculate_F2_self .................... src/AuxiliaryFunctions.jl:968-987
   print_log_data (in the MD loop) ...... src/MD.jl:610-650, crystallisation guard :711-713
   create_when_to_save_array ............ src/Dump.jl:18-31
+  save_data / prepare_savefile ......... src/Dump.jl:60-88, 121-156
+  compute_potential_energy ............. src/AuxiliaryFunctions.jl:186-196
+  read_restart ......................... src/Restart.jl:1-104
   find_inherent_structure .............. src/InherentStructure.jl:5-87
 """
+import json
 import math
+import os
 import time
+import unicodedata
 
 import numpy as np
 
@@ -460,25 +466,154 @@ def _run_device_steps(ctx, parameters, nsteps, noise, arrays, chunk=1000):
         done += k
 
 
+def compute_potential_energy(arrays, parameters, output, neighborlist):
+    """src/AuxiliaryFunctions.jl:186-196: potential_energy_array[i] =
+    find_energy(i) / 2, computed on the device."""
+    ctx = _bind(arrays, parameters, neighborlist)
+    arrays.potential_energy_array[...] = ctx.potential_energies()
+
+
+# ------------------------------------------------------------- dump files
+#
+# The reference writes one HDF5 file (src/Dump.jl): groups positions/,
+# velocities/, forces/, diameters/, potential_energy/, miscellaneous/ with
+# one dims x N FP64 dataset per saved step, plus the parameters as file
+# attributes.  No HDF5 library is available to this package, so the same
+# hierarchy is kept as a directory: <filename>/<group>/<step>.npy holds the
+# dataset (shape (dims, N) for vectors, element [c, i] = component c of
+# particle i, as reinterpret(reshape, Float64, ...) gives it) and
+# <filename>/attributes.json the attributes.
+
+def _attr_value(x):
+    if isinstance(x, (bool, int, float, str)) or x is None:
+        return x
+    if isinstance(x, (np.integer, np.floating)):
+        return x.item()
+    if isinstance(x, (set, frozenset, list, tuple, np.ndarray)):
+        return sorted(int(v) for v in x)
+    return type(x).__name__
+
+
+def prepare_savefile(parameters, arrays, neighborlist=None):
+    """src/Dump.jl:121-156: create the groups, store the parameter, system,
+    potential and dump_info fields as attributes and the diameters.  Unlike
+    the reference (whose restart then fails on the missing attribute,
+    src/Restart.jl:32) when_to_save_array is stored too, and so is the skin
+    distance when ``neighborlist`` is given (the reference's restart assumes
+    0.35, src/Restart.jl:102)."""
+    di = parameters.dump_info
+    if di is None or not di.save:
+        return
+    root = di.filename
+    for g in ("positions", "velocities", "forces", "diameters", "potential_energy", "miscellaneous"):
+        os.makedirs(os.path.join(root, g), exist_ok=True)
+    attrs = {}
+    for k, v in vars(parameters).items():
+        attrs[k] = _attr_value(v) if k not in ("system", "interaction_potential", "dump_info", "callback") \
+            else type(v).__name__
+    for obj in (parameters.system, parameters.interaction_potential, di):
+        for k, v in vars(obj).items():
+            attrs[k] = _attr_value(v)
+    if neighborlist is not None:
+        attrs["skin_distance"] = float(neighborlist.skin_distance)
+        attrs["maxneighbours"] = int(neighborlist.maxneighbours)
+    with open(os.path.join(root, "attributes.json"), "w") as f:
+        json.dump(attrs, f, indent=1, sort_keys=True)
+    np.save(os.path.join(root, "diameters", "diameters.npy"), arrays.D_array)
+
+
+def dump_writer(filename):
+    """A ``writer(step, group, array)`` for save_data that stores each dataset
+    as <filename>/<group>/<step>.npy."""
+    def write(step, group, array):
+        d = os.path.join(filename, group)
+        os.makedirs(d, exist_ok=True)
+        a = np.asarray(array)
+        np.save(os.path.join(d, f"{int(step)}.npy"), a.T.copy() if a.ndim == 2 else a)
+    return write
+
+
 def save_data(arrays, parameters, output, neighborlist, restarted, writer=None):
     """src/Dump.jl:60-88: at save steps the lists are rebuilt (:66-68) and the
-    requested arrays handed to ``writer(step, name, array)`` (HDF5 writing
-    itself is not part of this library)."""
+    requested arrays handed to ``writer(step, group, array)``; by default
+    (writer None) they go to the dump directory ``dump_info.filename``."""
     di = parameters.dump_info
     if di is None or not (di.save and output.steps_done in di.when_to_save_array):
         return
     if not restarted:
         update_neighbor_lists(arrays, parameters, output, neighborlist)
-    if writer is not None:
-        s = output.steps_done
-        if di.save_r:
-            writer(s, "positions", arrays.r_array.copy())
-        if di.save_v:
-            writer(s, "velocities", arrays.v_array.copy())
-        if di.save_F:
-            writer(s, "forces", arrays.F_array.copy())
-        if di.save_D:
-            writer(s, "diameters", arrays.D_array.copy())
+    if writer is None:
+        writer = dump_writer(di.filename)
+    s = output.steps_done
+    if di.save_r:
+        writer(s, "positions", arrays.r_array.copy())
+    if di.save_v:
+        writer(s, "velocities", arrays.v_array.copy())
+    if di.save_F:
+        writer(s, "forces", arrays.F_array.copy())
+    if di.save_D:
+        writer(s, "diameters", arrays.D_array.copy())
+    if di.save_Epot:
+        compute_potential_energy(arrays, parameters, output, neighborlist)
+        writer(s, "potential_energy", arrays.potential_energy_array.copy())
+
+
+def read_restart(file, timestep, *, precision="fp64", cell_divisor=1):
+    """src/Restart.jl:1-104: rebuild arrays, parameters, output and the
+    neighbour struct from a dump at ``timestep``.  Every potential (LJ,
+    Berthier) and system (Newtonian, Langevin, Brownian) is reconstructed
+    (the reference handles Berthier + Newtonian only, src/Restart.jl:58-71),
+    and the stored when_to_save_array and skin distance are used."""
+    with open(os.path.join(file, "attributes.json")) as f:
+        raw = json.load(f)
+
+    class _Attrs(dict):  # Python stores identifiers NFKC-normalised (ϵ -> ε): look keys up the same way
+        def __getitem__(self, k):
+            return dict.__getitem__(self, unicodedata.normalize("NFKC", k))
+
+        def get(self, k, d=None):
+            return dict.get(self, unicodedata.normalize("NFKC", k), d)
+
+    a = _Attrs((unicodedata.normalize("NFKC", k), v) for k, v in raw.items())
+    dims, N = int(a["dims"]), int(a["N"])
+
+    def vec(group):
+        x = np.load(os.path.join(file, group, f"{int(timestep)}.npy"))
+        return np.ascontiguousarray(x.T, dtype=np.float64)
+
+    arrays = ArrayStruct(N, dims)
+    arrays.r_array[...] = vec("positions")
+    arrays.r_old_array[...] = arrays.r_array
+    arrays.v_array[...] = vec("velocities")
+    arrays.F_array[...] = vec("forces")
+    arrays.D_array[...] = np.load(os.path.join(file, "diameters", "diameters.npy"))
+    if a["interaction_potential"] == "Berthier":
+        U = Berthier(n=int(a.get("n", 12)), rc=float(a["rc"]), σ_ratio=float(a["σ_ratio"]), ζ=float(a["ζ"]))
+    elif a["interaction_potential"] == "LJ":
+        U = LJ(a["ϵ"], a["σ"], a["rc"])
+    else:
+        raise ValueError(f"unknown interaction potential {a['interaction_potential']}")
+    kind = a["system"]
+    if kind == "Newtonian":
+        S = Newtonian(a["kBT"], a["Δt"], a["m"], a["rescale_temperature"], dims)
+    elif kind == "Langevin":
+        S = Langevin(a["kBT"], a["γ"], a["Δt"], a["m"], dims)
+    elif kind == "Brownian":
+        S = Brownian(a["kBT"], a["γ"], a["Δt"], dims)
+    else:
+        raise ValueError(f"unknown system {kind}")
+    di = DumpInfo(a["save"], a["filename"], set(a.get("when_to_save_array", [])), a["save_r"], a["save_v"],
+                  a["save_F"], a["save_D"], a["save_Epot"])
+    parameters = ParameterStruct(a["N_MD_equilibration_steps"], a["random_seed"], a["ρ"], N, a["box_size"],
+                                 a["N_steps"], a["swap_probability"], a["max_MC_displacement"],
+                                 a["force_cutoff2"], a["q_cutoff"], S, U, di, None)
+    output = OutputStruct()
+    output.steps_done = int(timestep)
+    neighborlist = initialize_neighbor_struct(float(a.get("skin_distance", 0.35)), parameters.box_size,
+                                              math.sqrt(parameters.force_cutoff2), N, arrays.D_array, dims,
+                                              maxneighbours=int(a.get("maxneighbours", 200)),
+                                              precision=precision, cell_divisor=cell_divisor)
+    return arrays, parameters, output, neighborlist
 
 
 def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, restarted=False,
@@ -504,7 +639,9 @@ def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, rest
         ctx.set_state(v=arrays.v_array)
     output.potential_energy = ctx.full_energy()
     output.N_neighbor_list_rebuilds = 0
-    if new_file:
+    if new_file:  # src/MD.jl:698-701
+        if writer is None:
+            prepare_savefile(parameters, arrays, neighborlist)
         save_data(arrays, parameters, output, neighborlist, restarted, writer)
     if parameters.callback is not None:
         parameters.callback(arrays, parameters, output, neighborlist)

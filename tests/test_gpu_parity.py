@@ -403,3 +403,28 @@ def test_large_system_forces_match_oracle():
     assert H.force_rel_err(g["F"], Fo) <= 1e-12
     # Newton's third law on the device result
     assert np.all(np.abs(g["F"].sum(axis=0)) <= 1e-12 * np.abs(g["F"]).sum())
+
+
+@pytest.mark.parametrize("cell_divisor", [1, 2])
+def test_forced_rebuild_keeps_stored_forces(cell_divisor):
+    """update_neighbor_lists! between steps leaves every particle's stored F
+    (the next half-kick's input) as it was, although the rebuild re-sorts
+    the slots; and a step after it matches the oracle."""
+    op, r, D, v, rc = H.make_system("berthier", 512, 3, seed=91)
+    ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor)
+    ctx.update_neighbor_lists()
+    ctx.do_time_steps(30)  # scramble the slot order away from the host order
+    st = ctx.get_state()
+    F = np.random.default_rng(3).standard_normal(st["F"].shape)
+    ctx.set_state(F=F)
+    ctx.update_neighbor_lists()
+    assert np.array_equal(ctx.get_state(r=False, v=False, F=True)["F"], F)
+    o = oracle.State(op, st["r"], st["v"], D, F=F)
+    o.update_neighbor_lists()
+    ctx.do_time_steps(5)
+    o.do_time_steps(5)
+    g = ctx.get_state()
+    dr = g["r"] - o.r
+    dr -= np.round(dr / op.box_size) * op.box_size
+    assert np.max(np.abs(dr)) <= 1e-10
+    assert np.max(np.abs(g["v"] - o.v)) <= 1e-10

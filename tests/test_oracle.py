@@ -315,3 +315,15 @@ def test_all_core_baseline_follows_the_serial_oracle(kind, dims, N):
         assert np.max(np.abs(dr)) <= 1e-9
         assert np.max(np.abs(vb - st.v)) <= 1e-9
         assert nreb == st.rebuilds[0] - 1
+
+
+@pytest.mark.parametrize("kind,dims", [("berthier", 3), ("lj", 3), ("berthier", 2), ("lj", 2)])
+def test_potential_energies_sum_to_the_full_energy(kind, dims):
+    """compute_potential_energy! halves each particle's find_energy, so the
+    per-particle energies add up to calculate_full_energy."""
+    op, r, D, v, rc = H.make_system(kind, 343 if dims == 3 else 400, dims, seed=31)
+    lists = oracle.build_lists(op, r, D, cells=False)
+    E = oracle.potential_energies(op, r, D, lists)
+    tot = oracle.full_energy(op, r, D, lists)
+    assert abs(E.sum() - tot) <= 1e-12 * max(1.0, abs(tot))
+    assert np.all(np.isfinite(E)) and np.any(E != 0.0)
