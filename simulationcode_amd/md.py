@@ -487,8 +487,10 @@ def compute_potential_energy(arrays, parameters, output, neighborlist):
 def _attr_value(x):
     if isinstance(x, (bool, int, float, str)) or x is None:
         return x
-    if isinstance(x, (np.integer, np.floating)):
+    if isinstance(x, (np.integer, np.floating, np.bool_)):
         return x.item()
+    if isinstance(x, os.PathLike):
+        return os.fspath(x)
     if isinstance(x, (set, frozenset, list, tuple, np.ndarray)):
         return sorted(int(v) for v in x)
     return type(x).__name__
@@ -504,7 +506,7 @@ def prepare_savefile(parameters, arrays, neighborlist=None):
     di = parameters.dump_info
     if di is None or not di.save:
         return
-    root = di.filename
+    root = os.fspath(di.filename)
     for g in ("positions", "velocities", "forces", "diameters", "potential_energy", "miscellaneous"):
         os.makedirs(os.path.join(root, g), exist_ok=True)
     attrs = {}
