@@ -354,7 +354,8 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, 1, c->s));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
-            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s));
+            // bookkeeping by the next step's force finish (one launch less per step)
+            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s, false));
         } else {
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_kick_drift_only(c->d_tiles, nt, c->d_rep, c->s));
@@ -367,6 +368,8 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
             if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, 1, c->s));
         }
     }
+    // the last Brownian step's rebuild bookkeeping (no next force finish)
+    if (c->h.sys == SC_SYS_BROWNIAN && nsteps > 0) KCHK(c, sc::launch_rebuild_done(c->d_rep, 1, c->s));
     KCHK(c, sc::launch_finalize(c->d_tiles, nt, c->d_rep, 1, c->s));
     HIPCHK(c, hipEventRecord(c->ev1, c->s));
     HIPCHK(c, hipEventSynchronize(c->ev1));

@@ -153,7 +153,9 @@ int enqueue_step(sc_batch *b)
         BK(b, sc::launch_drift_brownian(b->d_tiles, nt, b->d_reps, nr, b->s));
     }
     if (b->has_vv) BK(b, sc::launch_kick_drift(b->d_tiles, nt, b->d_reps, nr, b->s));
-    BK(b, sc::launch_rebuild(b->d_tiles, nt, b->d_reps, nr, b->geom, b->s, true));
+    // an all-Brownian batch leaves the rebuild bookkeeping to the next step's
+    // force finish (and the end of the call): one launch less per step
+    BK(b, sc::launch_rebuild(b->d_tiles, nt, b->d_reps, nr, b->geom, b->s, b->has_vv));
     if (b->has_vv) BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 1, 3, b->s));
     if (b->has_rescale) BK(b, sc::launch_ekin_com(b->d_tiles, nt, b->d_reps, nr, b->s));
     return SC_OK;
@@ -326,6 +328,7 @@ int sc_batch_step_async(sc_batch *b, int64_t nsteps)
         for (int64_t s = 0; s < nsteps; s++)
             if ((rc = enqueue_step(b))) return rc;
     }
+    if (!b->has_vv) BK(b, sc::launch_rebuild_done(b->d_reps, b->nact, b->s));
     BK(b, sc::launch_finalize(b->d_tiles, b->ntiles, b->d_reps, b->nact, b->s));
     BCHK(b, hipEventRecord(b->ev1, b->s));
     b->pending = nsteps;

@@ -142,6 +142,7 @@ constexpr int kBuildShapes[12][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {25
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
                    bool bookkeep = true);
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
+int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s);  // rebuild bookkeeping alone
 int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s);
 int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 // per (optional): E_i / 2 of every particle at its original index

@@ -2464,6 +2464,13 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
                 }
                 dyn->rebuild = 0;
                 dyn->pending = 1 | (R.rescale ? 2 : 0);
+            } else if (dyn->rebuild) {
+                // Brownian: bookkeeping of the previous step's rebuild (which
+                // ran after its drift; this force already used the new lists),
+                // before this step's drift raises the flag again
+                dyn->rebuilds += 1;
+                dyn->have_lists = 1;
+                dyn->rebuild = 0;
             }
             dyn->step += 1;
         }
@@ -2623,6 +2630,12 @@ int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, h
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_ekin_com, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
     fin(reps, nreps, FIN_EKCOM, SYS_NEWT, 0.0, s);
+    return check(hipGetLastError());
+}
+
+int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s)
+{
+    fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }
 
