@@ -91,7 +91,7 @@ def timed_run(ctx, steps, warmup):
     return {"wall_s": wall, "device_ms": ctx.last_elapsed_ms(), "rebuilds": ctx.rebuild_count() - reb0, "kt": kt}
 
 
-def drift_run(ctx, steps, samples=20):
+def drift_run(ctx, steps, samples=40):
     """Total-energy drift: least-squares slope of E over `steps` steps,
     relative to |E|, scaled to 10^5 steps; also the relative spread."""
     chunk = max(steps // samples, 1)
@@ -316,7 +316,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-copy-ceiling", action="store_true", help="large: skip the measured device-copy ceiling")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--drift-steps", type=int, default=4000)
+    ap.add_argument("--drift-steps", type=int, default=20000)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20261206)
     ap.add_argument("--cell-divisor", type=int, default=0,
