@@ -164,3 +164,28 @@ def test_md_restart_continues_bitwise(tmp_path):
     Eo = oracle.potential_energies(op, r100, D, lists)
     E = np.load(f"{fn}/potential_energy/100.npy")
     assert np.max(np.abs(E - Eo)) <= 1e-12 * max(1.0, np.max(np.abs(Eo)))
+
+
+def test_md_driver_trajectory_matches_oracle():
+    """perform_molecular_dynamics with save steps (each forcing a rebuild
+    between steps) and log steps follows the oracle running the same
+    sequence: steps, then update_neighbor_lists at every save step."""
+    op, r, D, v, rc = H.make_system("berthier", 512, 3, seed=83, kBT=0.2)
+    saves = [0, 40, 97, 150]
+    A, P, out, nl = _md_setup(r, D, op.box_size, 0.0, 150)
+    A.v_array[...] = v
+    got = {}
+    P.dump_info = md.DumpInfo(True, "unused", set(saves), True, False, False, False, False)
+    md.perform_molecular_dynamics(A, P, out, nl, equilibrate_velocities=False, log=None,
+                                  writer=lambda s, g, a: got.__setitem__(s, a))
+    st = oracle.State(op, r, v, D)  # F = 0 at the start, as the reference's first half-kick sees it
+    st.update_neighbor_lists()
+    done = 0
+    for s in saves[1:]:
+        st.do_time_steps(s - done)
+        done = s
+        st.update_neighbor_lists()
+        dr = got[s] - st.r
+        dr -= np.round(dr / op.box_size) * op.box_size
+        assert np.max(np.abs(dr)) <= 1e-10, s
+    assert np.max(np.abs(A.v_array - st.v)) <= 1e-10
