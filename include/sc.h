@@ -156,6 +156,17 @@ int sc_rebuild_count(sc_ctx *ctx, int64_t *out, int64_t set);
  * Returns the final max |F| in *maxforce (may be NULL). */
 int sc_minimize(sc_ctx *ctx, int64_t iters, double dt, double max_step, double *maxforce);
 
+/* find_inherent_structure's loop itself (src/InherentStructure.jl:19-56): the
+ * iterations of sc_minimize until max(max|F|, max|F*dt|) <= tolerance, the
+ * check made on the device after each iteration's move and rebuild, as the
+ * reference makes it; at most max_iters iterations.  *iterations gets the
+ * count done (the reference's `iteration`), *maxforce the max |F| of the last
+ * one and *converged 1 if the tolerance was met (0: the reference raises
+ * "Not converged").  F afterwards holds the last iteration's forces.  Any of
+ * the outputs may be NULL. */
+int sc_minimize_until(sc_ctx *ctx, int64_t max_iters, double dt, double max_step, double tolerance,
+                      int64_t *iterations, double *maxforce, int32_t *converged);
+
 /* The on-device noise generator's counter-based block function
  * (Philox4x32-10, Salmon et al., SC'11), exported so its known-answer
  * vectors can be checked on the host.  No reference counterpart: the

@@ -109,6 +109,10 @@ def lib():
             f.argtypes = [P, dp, dp, dp, ctypes.c_int, ctypes.c_double]
             f.restype = ctypes.c_double
         L.or_potential_energies.argtypes = [P, dp, dp, dp, dp, dp]
+        L.or_find_inherent_structure.restype = ctypes.c_int
+        L.or_find_inherent_structure.argtypes = ([P, ctypes.c_int] + [dp] * 10 +
+                                                 [ctypes.c_double, ctypes.c_int64, ctypes.c_double,
+                                                  ctypes.c_double, dp, dp, dp])
         L.or_baseline_steps.restype = ctypes.c_int
         L.or_baseline_steps.argtypes = [P, ctypes.c_int64, ctypes.c_int, dp, dp, dp, dp, dp]
         _lib = L
@@ -283,6 +287,23 @@ class State:
     def calculate_new_forces(self, noise=None):
         self.F = calculate_new_forces(self.p, self.r, self.D, self.lists, self.v, noise)
         return self.F
+
+    def find_inherent_structure(self, tolerance=1e-6, max_iterations=10 ** 6, max_step=0.001, dt=0.005,
+                                cells=True):
+        """src/InherentStructure.jl:4-56 on r in place (v, F, lists and
+        displacements mutated too).  Returns (iterations, max |F| of the last
+        iteration, converged); not converged after max_iterations iterations
+        is reported, not raised."""
+        it = np.zeros(1, dtype=np.int64)
+        fm = np.zeros(1)
+        rc = lib().or_find_inherent_structure(
+            ctypes.byref(self.p), 1 if cells else 0, _ptr(self.r), _ptr(self.D), _ptr(self.F), _ptr(self.v),
+            _ptr(self.disp), _ptr(self.disp2), _ptr(self.lists.nfull), _ptr(self.lists.full),
+            _ptr(self.lists.nhalf), _ptr(self.lists.half), float(tolerance), int(max_iterations),
+            float(max_step), float(dt), _ptr(it), _ptr(self.rebuilds), _ptr(fm))
+        if rc not in (0, -3):
+            raise RuntimeError(f"oracle minimiser failed: {rc}")
+        return int(it[0]), float(fm[0]), rc == 0
 
     def full_energy(self):
         return full_energy(self.p, self.r, self.D, self.lists)

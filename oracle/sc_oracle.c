@@ -44,6 +44,7 @@
 #define OR_OK 0
 #define OR_ERR_OVERFLOW -1
 #define OR_ERR_ARG -2
+#define OR_ERR_NOT_CONVERGED -3
 
 /* Mirrors ParameterStruct + System + InteractionPotential fields the hot path
  * reads (src/DataStructs.jl:10-19, 45-87, 138-153) plus the neighbour-struct
@@ -547,6 +548,71 @@ int or_update_neighbor_lists(const or_params *p, int use_cells, const double *r,
                              double *disp, double *disp2, int64_t *rebuilds)
 {
     return rebuild(p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+}
+
+/* find_inherent_structure -- src/InherentStructure.jl:4-56: capped steepest
+ * descent with the pair forces of calculate_new_forces!(arrays, parameters,
+ * neighborlist) (src/MD.jl:262-316, no thermostat terms).  Per iteration:
+ * v = F dt (:29-31); stepsize = max_i |v_i| (:33-36); v *= max_step/stepsize
+ * when stepsize > max_step (:38-40); move with displacement bookkeeping and
+ * floor wrap (:41-48); rebuild when max disp2 > (skin/2)^2 (:49-51);
+ * error = max(max_i |F_i|, stepsize) (:52-56), the loop ending once
+ * error <= tolerance.  Lists are rebuilt and displacements zeroed on entry
+ * (:8, :13-14).  Returns OR_OK, or OR_ERR_NOT_CONVERGED once iteration
+ * max_iterations+1 would start (:21-23).  r, F, v: AoS N x dims. */
+int or_find_inherent_structure(const or_params *p_in, int use_cells, double *r, const double *D, double *F, double *v,
+                               double *disp, double *disp2, int64_t *nfull, int64_t *full, int64_t *nhalf,
+                               int64_t *half, double tolerance, int64_t max_iterations, double max_step, double dt,
+                               int64_t *iterations, int64_t *rebuilds, double *maxforce)
+{
+    or_params p = *p_in;
+    p.system = OR_SYS_NEWTONIAN;          /* pair forces only */
+    int64_t N = p.N;
+    int dims = p.dims;
+    double thr = (p.skin_distance / 2.0) * (p.skin_distance / 2.0);
+    int rc = rebuild(&p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+    if (rc) return rc;
+    for (int64_t i = 0; i < N * dims; i++) disp[i] = 0.0;
+    for (int64_t i = 0; i < N; i++) disp2[i] = 0.0;
+    double err = tolerance * 2.0;
+    int64_t it = 0;
+    double fmax_last = 0.0;
+    while (err > tolerance) {
+        it++;
+        if (it > max_iterations) {
+            *iterations = it - 1;
+            *maxforce = fmax_last;
+            return OR_ERR_NOT_CONVERGED;
+        }
+        or_calculate_new_forces(&p, r, D, nhalf, half, NULL, NULL, F);
+        for (int64_t i = 0; i < N * dims; i++) v[i] = F[i] * dt;
+        double stepsize = 0.0;
+        for (int64_t i = 0; i < N; i++) {
+            double s = 0.0;
+            for (int c = 0; c < dims; c++) s = (c == 0) ? v[i * dims] * v[i * dims] : s + v[i * dims + c] * v[i * dims + c];
+            stepsize = fmax(stepsize, sqrt(s));
+        }
+        if (stepsize > max_step) {
+            double f = max_step / stepsize;
+            for (int64_t i = 0; i < N * dims; i++) v[i] *= f;
+        }
+        double mx = drift(&p, r, disp, disp2, v, 1.0);
+        if (mx > thr) {
+            rc = rebuild(&p, use_cells, r, D, nfull, full, nhalf, half, disp, disp2, rebuilds);
+            if (rc) return rc;
+        }
+        double fm = 0.0;
+        for (int64_t i = 0; i < N; i++) {
+            double s = 0.0;
+            for (int c = 0; c < dims; c++) s = (c == 0) ? F[i * dims] * F[i * dims] : s + F[i * dims + c] * F[i * dims + c];
+            fm = fmax(fm, sqrt(s));
+        }
+        fmax_last = fm;
+        err = fmax(fm, stepsize);
+    }
+    *iterations = it;
+    *maxforce = fmax_last;
+    return OR_OK;
 }
 
 /* ------------------------------------------------------------------------

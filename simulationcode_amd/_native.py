@@ -43,7 +43,7 @@ EXPORTS = (
     "sc_create", "sc_destroy", "sc_last_error", "sc_set_state", "sc_get_state",
     "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_calculate_new_forces",
     "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
-    "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_philox4x32_10",
+    "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_minimize_until", "sc_philox4x32_10",
     "sc_last_elapsed_ms", "sc_copy_bandwidth", "sc_potential_energies", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
     "sc_batch_create", "sc_batch_destroy", "sc_batch_last_error", "sc_replica_device_bytes",
     "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
@@ -152,6 +152,8 @@ def load(path=LIB_PATH):
         "sc_steinhardt": (ctypes.c_int, [vp, ctypes.c_int32, dbl, ctypes.c_int32, pd]),
         "sc_rebuild_count": (ctypes.c_int, [vp, ctypes.POINTER(i64), i64]),
         "sc_minimize": (ctypes.c_int, [vp, i64, dbl, dbl, pd]),
+        "sc_minimize_until": (ctypes.c_int, [vp, i64, dbl, dbl, dbl, ctypes.POINTER(i64), pd,
+                                             ctypes.POINTER(ctypes.c_int32)]),
         "sc_philox4x32_10": (None, [vp, vp, vp]),
         "sc_last_elapsed_ms": (ctypes.c_int, [vp, pd]),
         "sc_copy_bandwidth": (ctypes.c_int, [i64, ctypes.c_int32, pd]),
@@ -358,6 +360,16 @@ class Context:
         mf = ctypes.c_double()
         self._check(self._lib.sc_minimize(self._h, int(iters), float(dt), float(max_step), ctypes.byref(mf)))
         return mf.value
+
+    def minimize_until(self, max_iters, dt, max_step, tolerance):
+        """Iterates until max(max|F|, step) <= tolerance (checked on the
+        device after every move) or max_iters; returns (iterations, max |F|
+        of the last iteration, converged)."""
+        it, mf, cv = ctypes.c_int64(), ctypes.c_double(), ctypes.c_int32()
+        self._check(self._lib.sc_minimize_until(self._h, int(max_iters), float(dt), float(max_step),
+                                                float(tolerance), ctypes.byref(it), ctypes.byref(mf),
+                                                ctypes.byref(cv)))
+        return it.value, mf.value, bool(cv.value)
 
     def last_elapsed_ms(self):
         return self._scalar(self._lib.sc_last_elapsed_ms)

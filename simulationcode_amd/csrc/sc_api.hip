@@ -544,6 +544,62 @@ int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *ma
     return check_dyn(c);
 }
 
+// One minimiser iteration: force, step scale, move, skin-triggered rebuild.
+static int min_iteration(sc_ctx *c, double dt, double max_step)
+{
+    const int nt = c->ntiles;
+    if (sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 0, 7, c->s) ||
+        sc::launch_min_scale(c->d_tiles, nt, c->d_rep, 1, dt, max_step, c->s) ||
+        sc::launch_min_move(c->d_tiles, nt, c->d_rep, 1, dt, c->s) ||
+        sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s))
+        return fail(c, SC_ERR_HIP, std::string("kernel launch failed: ") + hipGetErrorString(hipGetLastError()));
+    return SC_OK;
+}
+
+int sc_minimize_until(sc_ctx *c, int64_t max_iters, double dt, double max_step, double tolerance,
+                      int64_t *iterations, double *maxforce, int32_t *converged)
+{
+    if (!c || !c->d_rep || max_iters < 0 || !(dt > 0.0) || !(max_step > 0.0) || !(tolerance >= 0.0))
+        return SC_ERR_ARG;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    // Each iteration checks max(max|F|, step) <= tolerance on the device after
+    // its move (FIN_MINSCALE); from then on the force, scale and move kernels
+    // of the remaining launched iterations return at once.  The host launches
+    // growing chunks and polls the stop state between them, so at most one
+    // chunk of no-op iterations is enqueued past convergence.
+    Rep saved = c->h;
+    c->h.sys = SC_SYS_NEWTONIAN;
+    c->h.min_tol = tolerance;
+    rc = push_rep(c);
+    if (!rc && sc::launch_min_mark(c->d_rep, 1, true, c->s))
+        rc = fail(c, SC_ERR_HIP, "kernel launch failed");
+    int64_t done = 0, chunk = 16;
+    bool conv = false;
+    while (!rc && done < max_iters) {
+        const int64_t k = std::min(chunk, max_iters - done);
+        for (int64_t it = 0; !rc && it < k; it++) rc = min_iteration(c, dt, max_step);
+        done += k;
+        if (!rc) rc = read_dyn(c, &d);
+        if (!rc) conv = d.min_state != 0;
+        if (rc || conv || d.err) break;
+        chunk = std::min<int64_t>(chunk * 2, 1024);
+    }
+    if (!rc && sc::launch_min_mark(c->d_rep, 1, false, c->s)) rc = fail(c, SC_ERR_HIP, "kernel launch failed");
+    c->host_step += done;
+    c->h = saved;
+    int rc2 = push_rep(c);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    if ((rc = read_dyn(c, &d))) return rc;
+    if (iterations) *iterations = d.min_iters;
+    if (maxforce) *maxforce = d.result[3];
+    if (converged) *converged = conv ? 1 : 0;
+    return check_dyn(c);
+}
+
 int sc_copy_bandwidth(int64_t bytes, int32_t reps, double *gbps)
 {
     if (bytes < 1024 || reps < 1 || !gbps) return SC_ERR_ARG;

@@ -53,6 +53,10 @@ struct RepDyn {
     unsigned int ctr[CTR_N];
     long long rebuilds;
     long long step;                   // force evaluations so far (noise counter)
+    int min_state;                    // minimiser run to a tolerance: 0 running, 1 converged in
+                                      // this iteration (its move still runs), 2 stopped
+    int min_pad;
+    long long min_iters;              // iterations done by the current tolerance run
     double vmean[3];
     double vscale;
     double Fmean[3];
@@ -84,6 +88,7 @@ struct Rep {
     double eps, lj_s2, lj_md, Uc;  // LJ: sigma^2 (specialised loops), find_mean_D = sigma^2
     double c0, c2, c4, zeta;
     double kBT, gamma, dt, m;
+    double min_tol;      // minimiser: stop once max |F| <= min_tol (< 0: never)
     uint64_t seed;
     // ---- injected noise (NULL => Philox): block s at noise + (step - noise_step0) * n * dims
     const double *noise;
@@ -161,6 +166,9 @@ int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, 
 int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, double max_step,
                      hipStream_t s);
 int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, hipStream_t s);
+// tolerance-run markers: begin resets the stop state and iteration count,
+// end unblocks the force kernel
+int launch_min_mark(const Rep *reps, int nreps, bool begin, hipStream_t s);
 }  // namespace sc
 
 // replica setup (sc_replica.hip)

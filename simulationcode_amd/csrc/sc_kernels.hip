@@ -146,9 +146,9 @@ __device__ inline int xcd_tile()
 // no atomics at all.
 __device__ inline bool need_rebuild(const Rep &R) { return gp(R.dyn)->rebuild != 0; }
 
-__device__ inline void publish_maxdisp(const Rep &R, double m)
+__device__ inline void publish_maxdisp(const Rep &R, double m, int bits = 1)
 {
-    if (threadIdx.x == 0 && m > R.thr) gatomic_or(&gp(R.dyn)->rebuild, 1);
+    if (threadIdx.x == 0 && m > R.thr) gatomic_or(&gp(R.dyn)->rebuild, bits);
 }
 
 // Per-tile partial slots (8 doubles per tile in Rep::part).
@@ -663,6 +663,7 @@ __global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *r
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     if (!((sysmask >> R.sys) & 1)) return;
+    if (R.min_tol >= 0.0 && gp(R.dyn)->min_state != 0) return;  // minimiser converged
     const int key = (R.dims == 3 ? 4 : 0) + (R.pot == 2 ? 2 : 0) + (R.prec ? 1 : 0);
     switch (key) {
     case 0: force_tile<2, 1, 0, MODE>(R, t); break;
@@ -2346,6 +2347,7 @@ __global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Re
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     SC_AS1 RepDyn *dyn = gp(R.dyn);
+    if (dyn->min_state != 0) return;  // converged (tolerance run)
     const int n = R.n, i = t.first + threadIdx.x;
     double st = 0.0, fm = 0.0;
     if (i < n) {
@@ -2369,6 +2371,7 @@ template <int DIMS>
 __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
 {
     SC_AS1 RepDyn *dyn = gp(R.dyn);
+    if (dyn->min_state == 2) return;  // converged and moved (tolerance run)
     const int n = R.n, i = t.first + threadIdx.x;
     const double scale = dyn->result[2];
     double s2 = 0.0;
@@ -2394,7 +2397,9 @@ __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
         s2 = s;
         if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }
-    publish_maxdisp(R, block_max(s2));
+    // F stays live across the rebuild: the forces of the last iteration are
+    // the minimiser's output beside the positions (src/InherentStructure.jl:25)
+    publish_maxdisp(R, block_max(s2), 1 | REBUILD_KEEP_F);
 }
 
 __global__ __launch_bounds__(SC_TB) void k_min_move(const Tile *tiles, const Rep *reps, double dt)
@@ -2423,6 +2428,8 @@ enum {
     FIN_EKIN,           // kinetic energy
     FIN_EKIN_RESCALE,   // + rescale_velocities! factor (pending)
     FIN_MINSCALE,       // minimiser step scale and max |F|
+    FIN_MIN_BEGIN,      // minimiser tolerance run: reset its state and count
+    FIN_MIN_END,        // ... and leave the force kernels unblocked
     FIN_MEAN            // compensated total / n
 };
 
@@ -2439,7 +2446,19 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     case FIN_DRIFT:
         break;
     case FIN_FORCE0:
-        if (tid == 0) dyn->step += 1;
+        if (tid == 0) {
+            dyn->step += 1;
+            if (dyn->min_state == 1) dyn->min_state = 2;  // the converged iteration's move is done
+        }
+        break;
+    case FIN_MIN_BEGIN:
+        if (tid == 0) {
+            dyn->min_state = 0;
+            dyn->min_iters = 0;
+        }
+        break;
+    case FIN_MIN_END:
+        if (tid == 0) dyn->min_state = 0;
         break;
     case FIN_FORCE1:
     case FIN_FORCE2: {
@@ -2529,6 +2548,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
         }
         break;
     case FIN_MINSCALE: {
+        if (dyn->min_state != 0) break;  // converged: the iteration is a no-op
         double st = 0.0, fm = 0.0;
         for (int q = tid; q < R.ntiles; q += SC_TB) {
             st = fmax(st, P[PS_AUXMAX * R.ntiles + q]);
@@ -2539,6 +2559,10 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
         if (tid == 0) {
             dyn->result[2] = (st > arg) ? arg / st : -1.0;  // -1: unscaled
             dyn->result[3] = fm;
+            dyn->min_iters += 1;
+            // src/InherentStructure.jl:50-54: error = max(max|F|, step) with the
+            // step taken before the cap (= max|F| dt < max|F|), checked after the move
+            if (R.min_tol >= 0.0 && fmax(fm, st) <= R.min_tol) dyn->min_state = 1;
         }
     } break;
     }
@@ -2815,6 +2839,12 @@ int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, int nreps, 
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_min_scale, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, dt);
     fin(reps, nreps, FIN_MINSCALE, SYS_ALL, max_step, s);
+    return check(hipGetLastError());
+}
+
+int launch_min_mark(const Rep *reps, int nreps, bool begin, hipStream_t s)
+{
+    fin(reps, nreps, begin ? FIN_MIN_BEGIN : FIN_MIN_END, SYS_ALL, 0.0, s);
     return check(hipGetLastError());
 }
 

@@ -168,6 +168,7 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     h.gamma = p->gamma;
     h.dt = p->dt;
     h.m = p->m;
+    h.min_tol = -1.0;
     h.seed = p->seed;
     long long nc;
     int srange;

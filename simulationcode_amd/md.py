@@ -699,21 +699,31 @@ def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, rest
 
 
 def find_inherent_structure(r_array_to_be_minimized, parameters, arrays, neighborlist, output, *,
-                            tolerance=1e-6, max_iterations=10 ** 6, max_step_size=0.001, dt=0.005,
-                            chunk=100):
-    """src/InherentStructure.jl:5-87: capped steepest descent on the device
-    (pair forces only), iterated in chunks until max(|F|, step) <= tolerance."""
-    arrays.r_array[...] = r_array_to_be_minimized
+                            tolerance=1e-6, max_iterations=10 ** 6, max_step_size=0.001, verbose=1, dt=0.005):
+    """src/InherentStructure.jl:4-87: capped steepest descent on the device
+    (pair forces only) until max(max|F|, step) <= tolerance, checked after
+    every iteration's move as the reference checks it; RuntimeError("Not
+    converged") past max_iterations.  arrays.r_array, F_array (the last
+    iteration's forces) and potential_energy_array are mutated, skin-triggered
+    rebuilds are counted in output.  verbose >= 1 prints the reference's
+    closing line (iterations, total potential energy, largest displacement);
+    verbose's per-1000-iteration report (>= 2) is not reproduced."""
+    r0 = np.array(r_array_to_be_minimized, dtype=np.float64, copy=True)
+    arrays.r_array[...] = r0
     update_neighbor_lists(arrays, parameters, output, neighborlist)
     ctx = neighborlist.ctx
-    it = 0
-    while True:
-        k = min(chunk, max_iterations - it)
-        if k <= 0:
-            raise RuntimeError("Not converged")
-        maxforce = ctx.minimize(k, dt, max_step_size)
-        it += k
-        if max(maxforce, min(maxforce * dt, max_step_size)) <= tolerance:
-            break
+    before = ctx.rebuild_count()
+    it, _, converged = ctx.minimize_until(max_iterations, dt, max_step_size, tolerance)
+    _sync_rebuilds(output, ctx, before)
     _download(arrays, neighborlist, ctx)
+    if not converged:
+        raise RuntimeError("Not converged")
+    compute_potential_energy(arrays, parameters, output, neighborlist)
+    if verbose >= 1:
+        L = parameters.box_size
+        d = arrays.r_array - r0
+        d -= np.round(d / L) * L
+        maxchange = float(np.sqrt(np.max(np.sum(d * d, axis=1)))) if len(d) else 0.0
+        print("after ", it, " iterations, the potential energy is ", float(np.sum(arrays.potential_energy_array)),
+              ". Maximal total displacement = ", maxchange, sep="")
     return arrays.r_array.copy()

@@ -327,3 +327,62 @@ def test_potential_energies_sum_to_the_full_energy(kind, dims):
     tot = oracle.full_energy(op, r, D, lists)
     assert abs(E.sum() - tot) <= 1e-12 * max(1.0, abs(tot))
     assert np.all(np.isfinite(E)) and np.any(E != 0.0)
+
+
+# ------------------------------------------------------- inherent structures
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_minimiser_capped_phase_moves_max_step_per_iteration(dims):
+    """LJ dimer at r = 1: |F dt| = 0.12 > max_step = 0.001, so each of the
+    first iterations moves both particles apart by exactly max_step along the
+    bond; the reported max |F| is the closed-form -dV/dr at the start of the
+    last iteration."""
+    op, r, D, lists, u = _two_particles(oracle.POT_LJ, dims, 1.0)
+    st = oracle.State(op, r, np.zeros_like(r), D)
+    it, fm, ok = st.find_inherent_structure(tolerance=1e-9, max_iterations=10, max_step=0.001, dt=0.005)
+    assert not ok and it == 10
+    d = st.r[1] - st.r[0]
+    assert abs(np.linalg.norm(d) - 1.02) < 1e-12
+    np.testing.assert_allclose(d / np.linalg.norm(d), u, atol=1e-12)
+    s = 1.018  # separation at the start of the tenth iteration
+    assert fm == pytest.approx(24.0 * (2.0 * s ** -13 - s ** -7), rel=1e-12)
+    np.testing.assert_allclose(st.r.mean(axis=0), r.mean(axis=0), atol=1e-12)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_minimiser_lj_dimer_settles_at_the_potential_minimum(dims):
+    """Run to tolerance, the dimer ends at the minimum of V = 4(r^-12 - r^-6),
+    r* = 2^(1/6), to within tol / V''(r*) (V''(r*) = 57.1); its centre of
+    mass stays put."""
+    op, r, D, lists, u = _two_particles(oracle.POT_LJ, dims, 1.0)
+    st = oracle.State(op, r, np.zeros_like(r), D)
+    it, fm, ok = st.find_inherent_structure(tolerance=1e-9, max_iterations=10000, max_step=0.001, dt=0.005)
+    assert ok and fm <= 1e-9 and 60 < it < 200
+    assert abs(np.linalg.norm(st.r[1] - st.r[0]) - 2.0 ** (1.0 / 6.0)) < 1e-10
+    np.testing.assert_allclose(st.r.mean(axis=0), r.mean(axis=0), atol=1e-12)
+
+
+@pytest.mark.parametrize("dims,N", [(3, 125), (2, 144)])
+def test_minimiser_descends_to_a_stationary_point(dims, N):
+    """Polydisperse soft spheres from an overlapping start: the energy falls
+    between checkpoints, the end state's forces recomputed from fresh lists
+    are below the tolerance, positions stay wrapped in [0, L), and
+    skin-triggered rebuilds happen."""
+    op, r, D, v, rc = H.make_system("berthier", N, dims, seed=31, relax=False)
+    tol = 1e-4
+    energies = []
+    for k in (100, 1000):
+        st = oracle.State(op, r, v, D)
+        it, fm, ok = st.find_inherent_structure(tolerance=tol, max_iterations=k, max_step=0.001, dt=0.005)
+        assert not ok and it == k
+        energies.append(oracle.full_energy(op, st.r, D, oracle.build_lists(op, st.r, D)))
+    st = oracle.State(op, r, v, D)
+    it, fm, ok = st.find_inherent_structure(tolerance=tol, max_iterations=100000, max_step=0.001, dt=0.005)
+    assert ok and it > 1000 and fm <= tol
+    assert st.rebuilds[0] > 1
+    lists = oracle.build_lists(op, st.r, D)
+    energies.append(oracle.full_energy(op, st.r, D, lists))
+    assert energies[0] > energies[1] > energies[2]
+    F = oracle.calculate_new_forces(op, st.r, D, lists)
+    assert np.max(np.linalg.norm(F, axis=1)) <= 2 * tol
+    assert np.all(st.r >= 0.0) and np.all(st.r < op.box_size)
