@@ -229,6 +229,14 @@ int sc_batch_step_async(sc_batch *b, int64_t nsteps);
 int sc_batch_wait(sc_batch *b);
 /* Capped steepest descent (sc_minimize) of every admitted replica. */
 int sc_batch_minimize(sc_batch *b, int64_t iters, double dt, double max_step);
+/* sc_minimize_until for every admitted replica at once (find_inherent_structure,
+ * src/InherentStructure.jl:19-56, per replica): each replica stops on its own
+ * iteration count, exactly as it would alone.  iterations[id] and
+ * converged[id] (arrays of max_replicas entries indexed by replica id, or
+ * NULL) receive each admitted replica's count and whether it met the
+ * tolerance within max_iters. */
+int sc_batch_minimize_until(sc_batch *b, int64_t max_iters, double dt, double max_step, double tolerance,
+                            int64_t *iterations, int32_t *converged);
 int sc_batch_get_state(sc_batch *b, int32_t id, double *r, double *v, double *F);
 /* Overwrite r, v and/or F of a replica (NULL = keep); lists are not rebuilt. */
 int sc_batch_set_state(sc_batch *b, int32_t id, const double *r, const double *v, const double *F);

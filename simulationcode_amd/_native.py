@@ -46,7 +46,7 @@ EXPORTS = (
     "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_minimize_until", "sc_philox4x32_10",
     "sc_last_elapsed_ms", "sc_copy_bandwidth", "sc_potential_energies", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
     "sc_batch_create", "sc_batch_destroy", "sc_batch_last_error", "sc_replica_device_bytes",
-    "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize",
+    "sc_batch_add", "sc_batch_remove", "sc_batch_set_graph", "sc_batch_step", "sc_batch_minimize", "sc_batch_minimize_until",
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
     "sc_batch_neighbor_stats", "sc_batch_replica_error",
     "sc_set_build_launch", "sc_batch_set_build_launch", "sc_autotune", "sc_batch_autotune",
@@ -170,6 +170,7 @@ def load(path=LIB_PATH):
         "sc_batch_set_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
         "sc_batch_step": (ctypes.c_int, [vp, i64]),
         "sc_batch_minimize": (ctypes.c_int, [vp, i64, dbl, dbl]),
+        "sc_batch_minimize_until": (ctypes.c_int, [vp, i64, dbl, dbl, dbl, vp, vp]),
         "sc_batch_get_state": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
         "sc_batch_set_state": (ctypes.c_int, [vp, ctypes.c_int32, vp, vp, vp]),
         "sc_batch_replica_stats": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.POINTER(i64), ctypes.POINTER(i64),
@@ -411,6 +412,7 @@ class Batch:
 
     def __init__(self, max_replicas=256, graph=False):
         self._lib = load()
+        self.max_replicas = int(max_replicas)
         h = ctypes.c_void_p()
         rc = self._lib.sc_batch_create(ctypes.byref(h), int(max_replicas))
         self._h = h
@@ -467,6 +469,15 @@ class Batch:
 
     def minimize(self, iters, dt, max_step):
         self._check(self._lib.sc_batch_minimize(self._h, int(iters), float(dt), float(max_step)))
+
+    def minimize_until(self, max_iters, dt, max_step, tolerance):
+        """Every replica descends until its own max(max|F|, step) <= tolerance
+        or max_iters; returns {replica id: (iterations, converged)}."""
+        it = np.full(self.max_replicas, -1, dtype=np.int64)
+        cv = np.zeros(self.max_replicas, dtype=np.int32)
+        self._check(self._lib.sc_batch_minimize_until(self._h, int(max_iters), float(dt), float(max_step),
+                                                      float(tolerance), it.ctypes.data, cv.ctypes.data))
+        return {rid: (int(it[rid]), bool(cv[rid])) for rid in self._shape if it[rid] >= 0}
 
     def get_state(self, rid):
         n, d = self._shape[rid]

@@ -588,12 +588,12 @@ int sc_minimize_until(sc_ctx *c, int64_t max_iters, double dt, double max_step, 
         chunk = std::min<int64_t>(chunk * 2, 1024);
     }
     if (!rc && sc::launch_min_mark(c->d_rep, 1, false, c->s)) rc = fail(c, SC_ERR_HIP, "kernel launch failed");
-    c->host_step += done;
     c->h = saved;
     int rc2 = push_rep(c);
     if (rc) return rc;
     if (rc2) return rc2;
     if ((rc = read_dyn(c, &d))) return rc;
+    c->host_step += d.min_iters;  // force evaluations done (the device's step counter)
     if (iterations) *iterations = d.min_iters;
     if (maxforce) *maxforce = d.result[3];
     if (converged) *converged = conv ? 1 : 0;

@@ -383,6 +383,64 @@ int sc_batch_minimize(sc_batch *b, int64_t iters, double dt, double max_step)
     return batch_errors(b);
 }
 
+int sc_batch_minimize_until(sc_batch *b, int64_t max_iters, double dt, double max_step, double tolerance,
+                            int64_t *iterations, int32_t *converged)
+{
+    if (!b || max_iters < 0 || !(dt > 0.0) || !(max_step > 0.0) || !(tolerance >= 0.0)) return SC_ERR_ARG;
+    int rc = sync_tables(b);
+    if (rc) return rc;
+    if (b->nact == 0) return SC_OK;
+    // the descriptors are re-uploaded from the slots by the next call if this
+    // one stops early (views left on the device)
+    b->dirty = true;
+    const int nt = b->ntiles, nr = b->nact;
+    std::vector<Rep> view(nr);
+    for (int r = 0; r < nr; r++) {
+        view[r] = b->slots[b->act[r]].h;
+        view[r].sys = SC_SYS_NEWTONIAN;
+        view[r].min_tol = tolerance;
+    }
+    BCHK(b, hipMemcpyAsync(b->d_reps, view.data(), sizeof(Rep) * nr, hipMemcpyHostToDevice, b->s));
+    BK(b, sc::launch_min_mark(b->d_reps, nr, true, b->s));
+    // every replica stops on its own (its kernels return once it has met the
+    // tolerance); the host polls the stop states between growing chunks
+    std::vector<int> st(nr, 0), er(nr, 0);
+    int64_t done = 0, chunk = 16;
+    while (done < max_iters) {
+        const int64_t k = std::min(chunk, max_iters - done);
+        for (int64_t it = 0; it < k; it++) {
+            BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 0, 7, b->s));
+            BK(b, sc::launch_min_scale(b->d_tiles, nt, b->d_reps, nr, dt, max_step, b->s));
+            BK(b, sc::launch_min_move(b->d_tiles, nt, b->d_reps, nr, dt, b->s));
+            BK(b, sc::launch_rebuild(b->d_tiles, nt, b->d_reps, nr, b->geom, b->s, true));
+        }
+        done += k;
+        BK(b, sc::launch_gather_err(b->d_reps, nr, b->d_err, b->s, 1));
+        BCHK(b, hipMemcpyAsync(st.data(), b->d_err, sizeof(int) * nr, hipMemcpyDeviceToHost, b->s));
+        BK(b, sc::launch_gather_err(b->d_reps, nr, b->d_err, b->s, 0));
+        BCHK(b, hipMemcpyAsync(er.data(), b->d_err, sizeof(int) * nr, hipMemcpyDeviceToHost, b->s));
+        BCHK(b, hipStreamSynchronize(b->s));
+        bool all = true;
+        for (int r = 0; r < nr; r++) all = all && (st[r] != 0 || er[r] != 0);
+        if (all) break;
+        chunk = std::min<int64_t>(chunk * 2, 1024);
+    }
+    BK(b, sc::launch_min_mark(b->d_reps, nr, false, b->s));
+    std::vector<RepDyn> dyn(nr);
+    for (int r = 0; r < nr; r++) {
+        view[r] = b->slots[b->act[r]].h;
+        BCHK(b, hipMemcpyAsync(&dyn[r], view[r].dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, b->s));
+    }
+    BCHK(b, hipMemcpyAsync(b->d_reps, view.data(), sizeof(Rep) * nr, hipMemcpyHostToDevice, b->s));
+    BCHK(b, hipStreamSynchronize(b->s));
+    b->dirty = false;
+    for (int r = 0; r < nr; r++) {
+        if (iterations) iterations[b->act[r]] = dyn[r].min_iters;
+        if (converged) converged[b->act[r]] = st[r] != 0 ? 1 : 0;
+    }
+    return batch_errors(b);
+}
+
 int sc_batch_get_state(sc_batch *b, int32_t id, double *r, double *v, double *F)
 {
     int rc = check_slot(b, id);

@@ -148,7 +148,8 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
                    bool bookkeep = true);
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
 int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s);  // rebuild bookkeeping alone
-int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s);
+// per replica: what = 0 its error bits, 1 its minimiser stop state
+int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s, int what = 0);
 int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 // per (optional): E_i / 2 of every particle at its original index
 int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, double *per = nullptr);

@@ -741,10 +741,14 @@ __global__ void k_request_rebuild(Rep *reps, int nreps)
 }
 
 // Error bits of every replica into one array (one host copy per check).
-__global__ void k_gather_err(const Rep *reps, int nreps, int *out)
+// what = 0: error bits; 1: minimiser stop state
+__global__ void k_gather_err(const Rep *reps, int nreps, int *out, int what)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nreps) out[r] = gp(reps[r].dyn)->err;
+    if (r < nreps) {
+        const SC_AS1 RepDyn *d = gp(reps[r].dyn);
+        out[r] = what == 0 ? d->err : d->min_state;
+    }
 }
 
 // Rebuild work list.  plan[0] = number of tiles to rebuild, plan[1] = number
@@ -2447,8 +2451,10 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
         break;
     case FIN_FORCE0:
         if (tid == 0) {
-            dyn->step += 1;
-            if (dyn->min_state == 1) dyn->min_state = 2;  // the converged iteration's move is done
+            // a stopped minimiser's force kernel returned at once: no
+            // evaluation to count (the step counter keys the noise streams)
+            if (dyn->min_state == 0) dyn->step += 1;
+            else if (dyn->min_state == 1) dyn->min_state = 2;  // the converged iteration's move is done
         }
         break;
     case FIN_MIN_BEGIN:
@@ -2669,10 +2675,10 @@ int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s)
     return check(hipGetLastError());
 }
 
-int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s)
+int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s, int what)
 {
     if (nreps <= 0) return 0;
-    hipLaunchKernelGGL(k_gather_err, dim3((nreps + SC_TB - 1) / SC_TB), dim3(SC_TB), 0, s, reps, nreps, out);
+    hipLaunchKernelGGL(k_gather_err, dim3((nreps + SC_TB - 1) / SC_TB), dim3(SC_TB), 0, s, reps, nreps, out, what);
     return check(hipGetLastError());
 }
 

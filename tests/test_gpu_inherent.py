@@ -106,3 +106,51 @@ def test_find_inherent_structure_mirror(capsys):
     A, P, out, nl = _md_objects(r, D, op.box_size, 3)
     with pytest.raises(RuntimeError, match="Not converged"):
         md.find_inherent_structure(r, P, A, nl, out, tolerance=1e-4, max_iterations=50, verbose=0)
+
+
+def test_batch_minimize_until_equals_each_replica_alone():
+    """sc_batch_minimize_until: every replica stops after the same number of
+    iterations, at bitwise the same positions and forces, as a single-system
+    context running sc_minimize_until on it alone; replicas converge at
+    different iterations, and replicas of the mix keep being stepped
+    correctly afterwards."""
+    from simulationcode_amd import workloads as W
+    specs = [
+        dict(N=700, dims=3, system="newtonian", kBT=0.2, dt=0.005, seed=1, precision="fp64"),
+        dict(N=900, dims=2, system="brownian", kBT=0.15, dt=0.001, seed=2, precision="mixed"),
+        dict(N=1500, dims=3, system="langevin", kBT=0.3, dt=0.005, seed=3, precision="fp64"),
+        dict(N=400, dims=3, system="newtonian", kBT=0.2, dt=0.005, seed=6, precision="fp64"),
+    ]
+    tol, max_it = 1e-3, 20000
+    b = nat.Batch(8, graph=True)
+    ids = []
+    for s in specs:
+        p, D, r, v = W.replica(**s)
+        ids.append(b.add(p, D, r, v))
+    b.minimize(60, 0.005, 0.02)
+    b.step(3)  # a captured step exists before the descent
+    res = b.minimize_until(max_it, DT, MAX_STEP, tol)
+    its = []
+    for k, s in enumerate(specs):
+        p, D, r, v = W.replica(**s)
+        ctx = nat.Context(p, D)
+        ctx.set_state(r=r, v=v)
+        ctx.update_neighbor_lists()
+        ctx.minimize(60, 0.005, 0.02)
+        ctx.do_time_steps(3)
+        it, fm, ok = ctx.minimize_until(max_it, DT, MAX_STEP, tol)
+        assert res[ids[k]] == (it, ok), (k, res[ids[k]], it, ok)
+        its.append(it)
+        g = ctx.get_state()
+        gb = b.get_state(ids[k])
+        for f in ("r", "F"):
+            assert np.array_equal(g[f], gb[f]), (k, f)
+        # stepping continues identically after the descent (the noise
+        # streams are keyed by the count of force evaluations)
+        ctx.do_time_steps(20)
+        specs[k]["_r"] = ctx.get_state()["r"]
+    assert len(set(its)) > 1
+    b.step(20)
+    for k in range(len(specs)):
+        assert np.array_equal(b.get_state(ids[k])["r"], specs[k]["_r"]), k
+    b.close()
