@@ -24,8 +24,10 @@ of steps (~15 s), rebuilds included.
 
 --workload batched: 256 independent replicas (N log-uniform in [500, 20000],
 75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian, per-replica
-seeds, mixed precision) advanced by one hipGraph-captured step; value =
-aggregate particle-steps/s.  dt = 1e-4: at the stated dt = 0.001 explicit
+seeds, mixed precision) split over two batches stepped concurrently on their
+own HIP streams, kernels launched eagerly (a captured hipGraph replay and a
+single stream were both measured slower); value = aggregate
+particle-steps/s over the wall time of the timed steps.  dt = 1e-4: at the stated dt = 0.001 explicit
 Euler-Maruyama blows up within 10^2-10^3 steps for this potential.
 --workload served: see simulationcode_amd/serve.py.
 """
@@ -256,7 +258,10 @@ def bench_batched(args, nat):
         cd, shape, tune = W.autotune_batch(specs, pilot=64, steps=60)
     log(f"[bench] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
         f"({time.perf_counter() - t0:.1f}s)")
-    b = nat.Batch(len(specs) + 1, graph=not args.no_graph)
+    if args.streams > 1:
+        b = nat.BatchGroup(len(specs) + 1, streams=args.streams, graph=args.graph)
+    else:
+        b = nat.Batch(len(specs) + 1, graph=args.graph)
     b.set_build_launch(*shape)
     ids, Ns, dims = [], [], []
     t0 = time.perf_counter()
@@ -294,7 +299,7 @@ def bench_batched(args, nat):
             "data": "synthetic (seeded lattices relaxed on device)",
             "config": {"workload": f"{len(ids)} independent polydisperse replicas, N log-uniform [500, 20000], "
                                    "75% 3D / 25% 2D, Brownian dt=1e-4, kBT U[0.1,0.3], per-replica seeds",
-                       "replicas": len(ids), "particles": tot, "graph": not args.no_graph,
+                       "replicas": len(ids), "particles": tot, "graph": args.graph, "streams": args.streams,
                        "cell_divisor": cd, "build_block": shape[0], "build_lanes_per_slot": shape[1],
                        "autotune": tune, "autotune_full_batch": full_tune},
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -315,7 +320,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-copy-ceiling", action="store_true", help="large: skip the measured device-copy ceiling")
-    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--graph", action="store_true",
+                    help="batched: replay the step as a captured hipGraph (eager launches measured faster)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="batched: split the replicas over this many batches on their own streams")
     ap.add_argument("--drift-steps", type=int, default=20000)
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--seed", type=int, default=20261206)

@@ -611,6 +611,12 @@ class BatchGroup:
         k, rid = self.where[gid]
         return self.batches[k].neighbor_stats(rid)
 
+    def autotune(self, steps=100):
+        """Tunes each batch's build launch shape on its own (one after the
+        other); returns the first batch's (table, chosen index)."""
+        res = [b.autotune(steps) for b in self.batches if b.info()["active"]]
+        return res[0]
+
     def info(self):
         infos = [b.info() for b in self.batches]
         return {"active": sum(i["active"] for i in infos), "particles": sum(i["particles"] for i in infos),

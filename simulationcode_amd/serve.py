@@ -43,7 +43,7 @@ class Job:
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes=64 << 30, max_replicas=512, max_chunk=1000, graph=True,
+    def __init__(self, hbm_budget_bytes=64 << 30, max_replicas=512, max_chunk=1000, graph=False,
                  keep_results=False, clock=time.perf_counter, cell_divisor=0, build_shape=(256, 4)):
         self.budget = int(hbm_budget_bytes)
         self.max_replicas = int(max_replicas)

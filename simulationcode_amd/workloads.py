@@ -99,7 +99,7 @@ def autotune_batch(specs, cell_divisors=(1, 2, 3), pilot=64, steps=100, relax=(3
     sub = specs[:pilot]
     table = []
     for cd in cell_divisors:
-        b = nat.Batch(len(sub) + 1, graph=True)
+        b = nat.Batch(len(sub) + 1)
         npart = 0
         for s in sub:
             p, D, r, v = replica(**dict(s, cell_divisor=cd))
