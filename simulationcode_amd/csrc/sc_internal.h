@@ -54,7 +54,8 @@ struct RepDyn {
     long long rebuilds;
     long long step;                   // force evaluations so far (noise counter)
     int min_state;                    // minimiser run to a tolerance: 0 running, 1 converged in
-                                      // this iteration (its move still runs), 2 stopped
+                                      // this iteration (its move still runs), 2 stopped; read
+                                      // only while Rep::min_tol >= 0 (a tolerance run)
     int min_pad;
     long long min_iters;              // iterations done by the current tolerance run
     double vmean[3];

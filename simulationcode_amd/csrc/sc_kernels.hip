@@ -2351,7 +2351,7 @@ __global__ __launch_bounds__(SC_TB) void k_min_scale(const Tile *tiles, const Re
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     SC_AS1 RepDyn *dyn = gp(R.dyn);
-    if (dyn->min_state != 0) return;  // converged (tolerance run)
+    if (R.min_tol >= 0.0 && dyn->min_state != 0) return;  // converged (tolerance run)
     const int n = R.n, i = t.first + threadIdx.x;
     double st = 0.0, fm = 0.0;
     if (i < n) {
@@ -2375,7 +2375,7 @@ template <int DIMS>
 __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
 {
     SC_AS1 RepDyn *dyn = gp(R.dyn);
-    if (dyn->min_state == 2) return;  // converged and moved (tolerance run)
+    if (R.min_tol >= 0.0 && dyn->min_state == 2) return;  // converged and moved (tolerance run)
     const int n = R.n, i = t.first + threadIdx.x;
     const double scale = dyn->result[2];
     double s2 = 0.0;
@@ -2453,7 +2453,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
         if (tid == 0) {
             // a stopped minimiser's force kernel returned at once: no
             // evaluation to count (the step counter keys the noise streams)
-            if (dyn->min_state == 0) dyn->step += 1;
+            if (R.min_tol < 0.0 || dyn->min_state == 0) dyn->step += 1;
             else if (dyn->min_state == 1) dyn->min_state = 2;  // the converged iteration's move is done
         }
         break;
@@ -2554,7 +2554,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
         }
         break;
     case FIN_MINSCALE: {
-        if (dyn->min_state != 0) break;  // converged: the iteration is a no-op
+        if (R.min_tol >= 0.0 && dyn->min_state != 0) break;  // converged: the iteration is a no-op
         double st = 0.0, fm = 0.0;
         for (int q = tid; q < R.ntiles; q += SC_TB) {
             st = fmax(st, P[PS_AUXMAX * R.ntiles + q]);
