@@ -6,6 +6,10 @@ not depend on the oracle build at run time.  Each fixture: seeded inputs
 (r, v, D, injected noise, parameters) and the oracle's outputs (half/full
 pair sets, forces, energies, state after 50 steps, rebuild count).
 
+The minimiser fixtures hold an overlapping start and the oracle's
+find_inherent_structure result (iteration count, positions, last forces,
+rebuild count).
+
 Run: python tests/golden/make_golden.py
 """
 import json
@@ -29,6 +33,13 @@ FIXTURES = [
 ]
 STEPS = 50
 PARAM_FIELDS = [f for f, _ in oracle.OrParams._fields_]
+# find_inherent_structure from an overlapping (unrelaxed) start:
+# name, dims, N, tolerance, max_iterations (dt 0.005, max step 0.001)
+MINIMISER_FIXTURES = [
+    ("minimiser_berthier3d_N125", 3, 125, 1e-4, 100000),
+    ("minimiser_berthier2d_N144", 2, 144, 1e-4, 100000),
+]
+MIN_DT, MIN_MAX_STEP = 0.005, 0.001
 
 
 def make(name, kind, dims, N, system):
@@ -53,6 +64,18 @@ def make(name, kind, dims, N, system):
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
 
 
+def make_minimiser(name, dims, N, tol, max_it):
+    op, r, D, v, rc = H.make_system("berthier", N, dims, seed=31, relax=False)
+    st = oracle.State(op, r, v, D)
+    it, fm, ok = st.find_inherent_structure(tolerance=tol, max_iterations=max_it, max_step=MIN_MAX_STEP, dt=MIN_DT)
+    assert ok
+    params = {f: getattr(op, f) for f in PARAM_FIELDS}
+    params["force_cutoff"] = rc
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), r=r, v=v, D=D, tolerance=tol, max_iterations=max_it,
+                        dt=MIN_DT, max_step=MIN_MAX_STEP, iterations=it, maxforce=fm, r_min=st.r, F_last=st.F,
+                        rebuilds=st.rebuilds[0], params=json.dumps(params))
+
+
 def load(name):
     z = np.load(os.path.join(HERE, name + ".npz"), allow_pickle=False)
     d = {k: z[k] for k in z.files}
@@ -70,4 +93,7 @@ def params_of(d):
 if __name__ == "__main__":
     for fx in FIXTURES:
         make(*fx)
+        print("wrote", fx[0])
+    for fx in MINIMISER_FIXTURES:
+        make_minimiser(*fx)
         print("wrote", fx[0])

@@ -60,3 +60,37 @@ def test_gpu_matches_fixture(name):
     assert np.max(np.abs(dr)) <= 1e-10
     assert np.max(np.abs(g["v"] - d["v_end"])) <= 1e-10
     assert ctx.rebuild_count() == d["rebuilds"]
+
+
+MIN_NAMES = [f[0] for f in G.MINIMISER_FIXTURES]
+
+
+@pytest.mark.parametrize("name", MIN_NAMES)
+def test_oracle_reproduces_minimiser_fixture(name):
+    d = G.load(name)
+    op = G.params_of(d)
+    st = oracle.State(op, d["r"], d["v"], d["D"])
+    it, fm, ok = st.find_inherent_structure(tolerance=float(d["tolerance"]), max_iterations=int(d["max_iterations"]),
+                                            max_step=float(d["max_step"]), dt=float(d["dt"]))
+    assert ok and it == d["iterations"] and fm == d["maxforce"]
+    assert np.array_equal(st.r, d["r_min"]) and np.array_equal(st.F, d["F_last"])
+    assert st.rebuilds[0] == d["rebuilds"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", MIN_NAMES)
+def test_gpu_matches_minimiser_fixture(name):
+    d = G.load(name)
+    op = G.params_of(d)
+    ctx = nat.Context(H.sc_params_from(op, d["params"]["force_cutoff"]), d["D"])
+    ctx.set_state(r=d["r"], v=d["v"])
+    ctx.update_neighbor_lists()
+    it, fm, ok = ctx.minimize_until(int(d["max_iterations"]), float(d["dt"]), float(d["max_step"]),
+                                    float(d["tolerance"]))
+    assert ok and it == d["iterations"]
+    assert ctx.rebuild_count() == d["rebuilds"]
+    g = ctx.get_state(r=True, F=True)
+    dr = g["r"] - d["r_min"]
+    dr -= np.round(dr / op.box_size) * op.box_size
+    assert np.max(np.abs(dr)) <= 1e-9
+    assert np.max(np.linalg.norm(g["F"] - d["F_last"], axis=1)) <= 1e-11
