@@ -154,3 +154,24 @@ def test_batch_minimize_until_equals_each_replica_alone():
     for k in range(len(specs)):
         assert np.array_equal(b.get_state(ids[k])["r"], specs[k]["_r"]), k
     b.close()
+
+
+def test_minimize_until_edge_cases():
+    """max_iters = 0 does nothing and reports not converged (the reference
+    raises before its first iteration); tolerance 0 is never met, every
+    iteration runs; an empty batch returns no results."""
+    op, r, D, v, rc = H.make_system("berthier", 125, 3, seed=31, relax=False)
+    ctx = _ctx(op, rc, r, v, D)
+    r0 = ctx.get_state(r=True)["r"]
+    it, fm, ok = ctx.minimize_until(0, DT, MAX_STEP, 1e-4)
+    assert it == 0 and not ok
+    assert np.array_equal(ctx.get_state(r=True)["r"], r0)
+    it, fm, ok = ctx.minimize_until(40, DT, MAX_STEP, 0.0)
+    assert it == 40 and not ok and fm > 0.0
+    b = nat.Batch(4)
+    assert b.minimize_until(10, DT, MAX_STEP, 1e-4) == {}
+    from simulationcode_amd import workloads as W
+    p, Dr, rr, vr = W.replica(N=300, dims=2, system="newtonian", kBT=0.2, dt=0.005, seed=9, precision="fp64")
+    rid = b.add(p, Dr, rr, vr)
+    assert b.minimize_until(25, DT, MAX_STEP, 0.0) == {rid: (25, False)}
+    b.close()
