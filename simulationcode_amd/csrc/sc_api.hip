@@ -161,6 +161,90 @@ int ensure_events(sc_ctx *c, size_t n)
     return SC_OK;
 }
 
+// Newtonian/Langevin steps enqueued in chunks ahead of the skin-trigger
+// decision, without the five rebuild launches of every step: a step whose
+// drift fires the trigger halts the chunk on the device (its force, its
+// finish and every later kernel of the chunk return at once, RepDyn::halt),
+// the host reads the state after the chunk, runs the rebuild, that step's
+// force and finish, and continues.  The kernels and their order per step are
+// those of the ordinary loop, so the trajectory is bitwise the same.  Chunk
+// length: what is left of the last rebuild interval, at least 1, at most 64
+// (measured at N=10^6: the early-returning launches a halt leaves in the
+// queue cost more than the extra host round trip of a chunk that ends just
+// short of the trigger).  Event slots from `base`
+// (timing): 0-1 the rebuild step's force, then three per chunk step.
+int steps_ahead(sc_ctx *c, int64_t nsteps, bool timed, size_t base)
+{
+    const int nt = c->ntiles;
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    long long step0 = d.step;
+    int64_t left = nsteps, since = 0, interval = 12;
+    bool rec_pending = false;
+    auto take_rec = [&]() -> int {
+        if (!rec_pending) return SC_OK;
+        float a = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, c->kev[base], c->kev[base + 1]));
+        c->t_force += a;
+        c->n_force++;
+        rec_pending = false;
+        return SC_OK;
+    };
+    while (left > 0) {
+        const int64_t m = std::min<int64_t>(std::min<int64_t>(std::max<int64_t>(interval - since, 1), 64), left);
+        if (timed && (rc = ensure_events(c, base + 2 + 3 * (size_t)m))) return rc;
+        for (int64_t i = 0; i < m; i++) {
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 2 + 3 * i], c->s));
+            KCHK(c, sc::launch_kick_drift_only(c->d_tiles, nt, c->d_rep, c->s, true));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 3 + 3 * i], c->s));
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 1, 7, c->s, false, true));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 4 + 3 * i], c->s));
+            KCHK(c, sc::launch_force_finish(c->d_rep, 1, 1, 7, c->s, true));
+        }
+        if ((rc = read_dyn(c, &d))) return rc;
+        if ((rc = take_rec())) return rc;
+        const int64_t done = (int64_t)(d.step - step0);
+        if (done < 0 || done > m || (!d.halt && done != m)) return fail(c, SC_ERR_STATE, "step chunk out of step");
+        if (timed) {
+            for (int64_t i = 0; i < m && i <= done; i++) {
+                if (i == done && !d.halt) break;
+                float a = 0.f, b = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&b, c->kev[base + 2 + 3 * i], c->kev[base + 3 + 3 * i]));
+                c->t_drift += b;
+                c->n_drift++;
+                if (i < done) {
+                    HIPCHK(c, hipEventElapsedTime(&a, c->kev[base + 3 + 3 * i], c->kev[base + 4 + 3 * i]));
+                    c->t_force += a;
+                    c->n_force++;
+                }
+            }
+        }
+        left -= done;
+        since += done;
+        step0 = d.step;
+        if (d.err) break;
+        if (d.halt) {
+            // step done+1 drifted and fired the trigger
+            KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s, false));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[base], c->s));
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 1, 7, c->s, false, false));
+            if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 1], c->s));
+            KCHK(c, sc::launch_force_finish(c->d_rep, 1, 1, 7, c->s, false));
+            rec_pending = timed;
+            left -= 1;
+            interval = since + 1;
+            since = 0;
+            step0 += 1;
+        }
+    }
+    if (rec_pending) {
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        if ((rc = take_rec())) return rc;
+    }
+    return SC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -347,7 +431,12 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     if (timed && (rc = ensure_events(c, 4 * (size_t)nsteps))) return rc;
     size_t ev = 0;
     HIPCHK(c, hipEventRecord(c->ev0, c->s));
-    for (int64_t s = 0; s < nsteps; s++) {
+    // Newtonian/Langevin without rescaling: chunks enqueued ahead of the
+    // rebuild decision (steps_ahead); a rebuild requested before this call
+    // goes through one ordinary step first
+    const bool ahead = c->h.sys != SC_SYS_BROWNIAN && !c->h.rescale;
+    const int64_t plain = ahead ? ((d.rebuild || d.halt) ? std::min<int64_t>(1, nsteps) : 0) : nsteps;
+    for (int64_t s = 0; s < plain; s++) {
         if (c->h.sys == SC_SYS_BROWNIAN) {
             if (timed) HIPCHK(c, hipEventRecord(c->kev[ev++], c->s));
             KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 2, 7, c->s));
@@ -368,6 +457,7 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
             if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, 1, c->s));
         }
     }
+    if (ahead && nsteps > plain && (rc = steps_ahead(c, nsteps - plain, timed, ev))) return rc;
     // the last Brownian step's rebuild bookkeeping (no next force finish)
     if (c->h.sys == SC_SYS_BROWNIAN && nsteps > 0) KCHK(c, sc::launch_rebuild_done(c->d_rep, 1, c->s));
     KCHK(c, sc::launch_finalize(c->d_tiles, nt, c->d_rep, 1, c->s));

@@ -56,7 +56,9 @@ struct RepDyn {
     int min_state;                    // minimiser run to a tolerance: 0 running, 1 converged in
                                       // this iteration (its move still runs), 2 stopped; read
                                       // only while Rep::min_tol >= 0 (a tolerance run)
-    int min_pad;
+    int halt;                         // step chunk launched ahead: a step's drift fired the skin
+                                      // trigger, so its force and every later kernel of the chunk
+                                      // return at once until the host has run the rebuild
     long long min_iters;              // iterations done by the current tolerance run
     double vmean[3];
     double vscale;
@@ -119,10 +121,12 @@ struct Tile {
 namespace sc {
 int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_copy(const double4 *src, double4 *dst, size_t n, hipStream_t s);
+// ahead = true: launched as part of a chunk of steps enqueued ahead of the
+// rebuild decision (RepDyn::halt)
 int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s,
-                 bool finish = true);
-int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s);
-int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s);
+                 bool finish = true, bool ahead = false);
+int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s, bool ahead = false);
+int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s, bool ahead = false);
 int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s);
 int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);

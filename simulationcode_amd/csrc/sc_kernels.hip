@@ -457,11 +457,13 @@ __device__ void kick_drift_tile(const Rep &R, const Tile &t)
     publish_maxdisp(R, block_max(s2));
 }
 
-__global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const Rep *reps)
+__global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const Rep *reps, int ahead)
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     if (R.sys == 2) return;
+    // halt is written only by the force finish, never during this launch
+    if (ahead && gp(R.dyn)->halt) return;
     if (R.dims == 3)
         kick_drift_tile<3>(R, t);
     else
@@ -658,11 +660,14 @@ __device__ void force_tile(const Rep &R, const Tile &t)
 }
 
 template <int MODE>
-__global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *reps, int sysmask)
+__global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *reps, int sysmask, int ahead)
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     if (!((sysmask >> R.sys) & 1)) return;
+    // launched ahead: this step's drift fired the trigger (the rebuild must
+    // come first) or an earlier one did (the chunk has halted)
+    if (ahead && (gp(R.dyn)->halt || gp(R.dyn)->rebuild)) return;
     if (R.min_tol >= 0.0 && gp(R.dyn)->min_state != 0) return;  // minimiser converged
     const int key = (R.dims == 3 ? 4 : 0) + (R.pot == 2 ? 2 : 0) + (R.prec ? 1 : 0);
     switch (key) {
@@ -2434,7 +2439,8 @@ enum {
     FIN_MINSCALE,       // minimiser step scale and max |F|
     FIN_MIN_BEGIN,      // minimiser tolerance run: reset its state and count
     FIN_MIN_END,        // ... and leave the force kernels unblocked
-    FIN_MEAN            // compensated total / n
+    FIN_MEAN,           // compensated total / n
+    FIN_FORCE1_AHEAD    // FIN_FORCE1 of a step launched ahead: halts the chunk instead when the trigger fired
 };
 
 __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, int op, int sysmask, double arg)
@@ -2446,6 +2452,15 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int tid = threadIdx.x;
     const SC_AS1 double *P = gp(R.part);
+    if (op == FIN_FORCE1_AHEAD) {
+        const int h = dyn->halt, rb = dyn->rebuild;
+        __syncthreads();  // every thread has read the flags before thread 0 may set halt
+        if (h || rb) {
+            if (!h && tid == 0) dyn->halt = 1;  // the host runs the rebuild, this force and finish
+            return;
+        }
+        op = FIN_FORCE1;
+    }
     switch (op) {
     case FIN_DRIFT:
         break;
@@ -2488,6 +2503,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
                     dyn->have_lists = 1;
                 }
                 dyn->rebuild = 0;
+                dyn->halt = 0;
                 dyn->pending = 1 | (R.rescale ? 2 : 0);
             } else if (dyn->rebuild) {
                 // Brownian: bookkeeping of the previous step's rebuild (which
@@ -2584,7 +2600,6 @@ inline int check(hipError_t e) { return e == hipSuccess ? 0 : -5; }
 namespace sc {
 
 constexpr int SYS_ALL = 7, SYS_VV = 3, SYS_BD = 4, SYS_NEWT = 1;
-int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s);
 
 static inline void fin(const Rep *reps, int nreps, int op, int mask, double arg, hipStream_t s)
 {
@@ -2608,34 +2623,36 @@ int launch_copy(const double4 *src, double4 *dst, size_t n, hipStream_t s)
 int launch_kick_drift(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, 0);
     return check(hipGetLastError());
 }
 
 int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s,
-                 bool finish)
+                 bool finish, bool ahead)
 {
     if (ntiles <= 0) return 0;
+    const int a = ahead ? 1 : 0;
     if (mode == 0)
-        hipLaunchKernelGGL(k_force<0>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask);
+        hipLaunchKernelGGL(k_force<0>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask, a);
     else if (mode == 1)
-        hipLaunchKernelGGL(k_force<1>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask);
+        hipLaunchKernelGGL(k_force<1>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask, a);
     else
-        hipLaunchKernelGGL(k_force<2>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask);
-    if (finish) return launch_force_finish(reps, nreps, mode, sysmask, s);
+        hipLaunchKernelGGL(k_force<2>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, sysmask, a);
+    if (finish) return launch_force_finish(reps, nreps, mode, sysmask, s, ahead);
     return check(hipGetLastError());
 }
 
-int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s)
+int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s, bool ahead)
 {
-    fin(reps, nreps, mode == 0 ? FIN_FORCE0 : (mode == 1 ? FIN_FORCE1 : FIN_FORCE2), sysmask, 0.0, s);
+    const int op = mode == 0 ? FIN_FORCE0 : (mode == 1 ? (ahead ? FIN_FORCE1_AHEAD : FIN_FORCE1) : FIN_FORCE2);
+    fin(reps, nreps, op, sysmask, 0.0, s);
     return check(hipGetLastError());
 }
 
-int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s)
+int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s, bool ahead)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, ahead ? 1 : 0);
     return check(hipGetLastError());
 }
 
