@@ -41,6 +41,7 @@ struct sc_ctx {
     int *d_nq = nullptr;
     double t_force = 0.0, t_drift = 0.0;
     long long n_force = 0, n_drift = 0;
+    RepDyn *h_dyn = nullptr;  // pinned copy of the device state, polled between step chunks
 };
 
 namespace {
@@ -176,8 +177,19 @@ int ensure_events(sc_ctx *c, size_t n)
 int steps_ahead(sc_ctx *c, int64_t nsteps, bool timed, size_t base)
 {
     const int nt = c->ntiles;
+    if (!c->h_dyn && hipHostMalloc((void **)&c->h_dyn, sizeof(RepDyn), hipHostMallocDefault) != hipSuccess) {
+        c->h_dyn = nullptr;
+        return fail(c, SC_ERR_OOM, "hipHostMalloc failed");
+    }
+    // the state read after each chunk: one copy into pinned memory, one wait
+    auto poll = [&](RepDyn *out) -> int {
+        HIPCHK(c, hipMemcpyAsync(c->h_dyn, c->h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(c, hipStreamSynchronize(c->s));
+        *out = *c->h_dyn;
+        return SC_OK;
+    };
     RepDyn d;
-    int rc = read_dyn(c, &d);
+    int rc = poll(&d);
     if (rc) return rc;
     long long step0 = d.step;
     int64_t left = nsteps, since = 0, interval = 12;
@@ -202,7 +214,7 @@ int steps_ahead(sc_ctx *c, int64_t nsteps, bool timed, size_t base)
             if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 4 + 3 * i], c->s));
             KCHK(c, sc::launch_force_finish(c->d_rep, 1, 1, 7, c->s, true));
         }
-        if ((rc = read_dyn(c, &d))) return rc;
+        if ((rc = poll(&d))) return rc;
         if ((rc = take_rec())) return rc;
         const int64_t done = (int64_t)(d.step - step0);
         if (done < 0 || done > m || (!d.halt && done != m)) return fail(c, SC_ERR_STATE, "step chunk out of step");
@@ -289,6 +301,7 @@ void sc_destroy(sc_ctx *c)
     if (c->s) (void)hipStreamSynchronize(c->s);
     for (void *q : c->allocs) (void)hipFree(q);
     if (c->d_noise) (void)hipFree(c->d_noise);
+    if (c->h_dyn) (void)hipHostFree(c->h_dyn);
     for (hipEvent_t e : c->kev) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
