@@ -162,12 +162,16 @@ int ensure_events(sc_ctx *c, size_t n)
     return SC_OK;
 }
 
-// Newtonian/Langevin steps enqueued in chunks ahead of the skin-trigger
-// decision, without the five rebuild launches of every step: a step whose
+// Steps enqueued in chunks ahead of the skin-trigger decision, without the
+// five rebuild launches of every step.  Newtonian/Langevin: a step whose
 // drift fires the trigger halts the chunk on the device (its force, its
 // finish and every later kernel of the chunk return at once, RepDyn::halt),
 // the host reads the state after the chunk, runs the rebuild, that step's
-// force and finish, and continues.  The kernels and their order per step are
+// force and finish, and continues.  Brownian (force, finish, drift per step;
+// not with kernel timing): the force after a drift that fired the trigger
+// halts the chunk; the host runs the rebuild (with its bookkeeping) and
+// continues from that step, and also rebuilds when a chunk's last drift
+// fired it.  The kernels and their order per step are
 // those of the ordinary loop, so the trajectory is bitwise the same.  Chunk
 // length: what is left of the last rebuild interval, at least 1, at most 64
 // (measured at N=10^6: the early-returning launches a halt leaves in the
@@ -203,10 +207,17 @@ int steps_ahead(sc_ctx *c, int64_t nsteps, bool timed, size_t base)
         rec_pending = false;
         return SC_OK;
     };
+    const bool bd = c->h.sys == SC_SYS_BROWNIAN;
+    if (bd && timed) return fail(c, SC_ERR_ARG, "Brownian step chunks are not timed");
     while (left > 0) {
         const int64_t m = std::min<int64_t>(std::min<int64_t>(std::max<int64_t>(interval - since, 1), 64), left);
         if (timed && (rc = ensure_events(c, base + 2 + 3 * (size_t)m))) return rc;
-        for (int64_t i = 0; i < m; i++) {
+        for (int64_t i = 0; bd && i < m; i++) {
+            KCHK(c, sc::launch_force(c->d_tiles, nt, c->d_rep, 1, 2, 7, c->s, false, true));
+            KCHK(c, sc::launch_force_finish(c->d_rep, 1, 2, 7, c->s, true));
+            KCHK(c, sc::launch_drift_brownian(c->d_tiles, nt, c->d_rep, 1, c->s, true));
+        }
+        for (int64_t i = 0; !bd && i < m; i++) {
             if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 2 + 3 * i], c->s));
             KCHK(c, sc::launch_kick_drift_only(c->d_tiles, nt, c->d_rep, c->s, true));
             if (timed) HIPCHK(c, hipEventRecord(c->kev[base + 3 + 3 * i], c->s));
@@ -236,6 +247,15 @@ int steps_ahead(sc_ctx *c, int64_t nsteps, bool timed, size_t base)
         since += done;
         step0 = d.step;
         if (d.err) break;
+        if (bd) {
+            // halted at the force after a firing drift, or the last drift fired
+            if (d.halt || d.rebuild) {
+                KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s, true));
+                interval = since;
+                since = 0;
+            }
+            continue;
+        }
         if (d.halt) {
             // step done+1 drifted and fired the trigger
             KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s, false));
@@ -447,7 +467,7 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     // Newtonian/Langevin without rescaling: chunks enqueued ahead of the
     // rebuild decision (steps_ahead); a rebuild requested before this call
     // goes through one ordinary step first
-    const bool ahead = c->h.sys != SC_SYS_BROWNIAN && !c->h.rescale;
+    const bool ahead = c->h.sys == SC_SYS_BROWNIAN ? !timed : !c->h.rescale;
     const int64_t plain = ahead ? ((d.rebuild || d.halt) ? std::min<int64_t>(1, nsteps) : 0) : nsteps;
     for (int64_t s = 0; s < plain; s++) {
         if (c->h.sys == SC_SYS_BROWNIAN) {
@@ -470,7 +490,12 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
             if (c->h.rescale) KCHK(c, sc::launch_ekin_com(c->d_tiles, nt, c->d_rep, 1, c->s));
         }
     }
-    if (ahead && nsteps > plain && (rc = steps_ahead(c, nsteps - plain, timed, ev))) return rc;
+    if (ahead && nsteps > plain) {
+        // a Brownian step ends with its rebuild, whose bookkeeping the next
+        // force finish would do: do it now, the chunks expect a clear flag
+        if (plain > 0 && c->h.sys == SC_SYS_BROWNIAN) KCHK(c, sc::launch_rebuild_done(c->d_rep, 1, c->s));
+        if ((rc = steps_ahead(c, nsteps - plain, timed, ev))) return rc;
+    }
     // the last Brownian step's rebuild bookkeeping (no next force finish)
     if (c->h.sys == SC_SYS_BROWNIAN && nsteps > 0) KCHK(c, sc::launch_rebuild_done(c->d_rep, 1, c->s));
     KCHK(c, sc::launch_finalize(c->d_tiles, nt, c->d_rep, 1, c->s));

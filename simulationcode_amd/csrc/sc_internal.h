@@ -128,7 +128,7 @@ int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int 
 int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s, bool ahead = false);
 int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s, bool ahead = false);
 int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s);
-int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
+int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, bool ahead = false);
 int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 struct RebuildGeom {
     int max_chunks = 0;      // max over replicas of ceil(ncell / SC_CHUNK)

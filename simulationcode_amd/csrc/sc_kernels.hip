@@ -505,11 +505,12 @@ __device__ void drift_brownian_tile(const Rep &R, const Tile &t)
     publish_maxdisp(R, block_max(s2));
 }
 
-__global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, const Rep *reps)
+__global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, const Rep *reps, int ahead)
 {
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     if (R.sys != 2) return;
+    if (ahead && gp(R.dyn)->halt) return;  // halt is written only by the force finish
     if (R.dims == 3)
         drift_brownian_tile<3>(R, t);
     else
@@ -2440,7 +2441,8 @@ enum {
     FIN_MIN_BEGIN,      // minimiser tolerance run: reset its state and count
     FIN_MIN_END,        // ... and leave the force kernels unblocked
     FIN_MEAN,           // compensated total / n
-    FIN_FORCE1_AHEAD    // FIN_FORCE1 of a step launched ahead: halts the chunk instead when the trigger fired
+    FIN_FORCE1_AHEAD,   // FIN_FORCE1 of a step launched ahead: halts the chunk instead when the trigger fired
+    FIN_FORCE2_AHEAD    // the same for FIN_FORCE2
 };
 
 __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, int op, int sysmask, double arg)
@@ -2452,14 +2454,14 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     SC_AS1 RepDyn *dyn = gp(R.dyn);
     const int tid = threadIdx.x;
     const SC_AS1 double *P = gp(R.part);
-    if (op == FIN_FORCE1_AHEAD) {
+    if (op == FIN_FORCE1_AHEAD || op == FIN_FORCE2_AHEAD) {
         const int h = dyn->halt, rb = dyn->rebuild;
         __syncthreads();  // every thread has read the flags before thread 0 may set halt
         if (h || rb) {
-            if (!h && tid == 0) dyn->halt = 1;  // the host runs the rebuild, this force and finish
+            if (!h && tid == 0) dyn->halt = 1;  // the host runs the rebuild and this step again
             return;
         }
-        op = FIN_FORCE1;
+        op = op == FIN_FORCE1_AHEAD ? FIN_FORCE1 : FIN_FORCE2;
     }
     switch (op) {
     case FIN_DRIFT:
@@ -2546,6 +2548,7 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
                 dyn->have_lists = 1;
             }
             dyn->rebuild = 0;
+            dyn->halt = 0;
             dyn->pending = 0;
         }
         break;
@@ -2644,7 +2647,8 @@ int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int 
 
 int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s, bool ahead)
 {
-    const int op = mode == 0 ? FIN_FORCE0 : (mode == 1 ? (ahead ? FIN_FORCE1_AHEAD : FIN_FORCE1) : FIN_FORCE2);
+    const int op = mode == 0 ? FIN_FORCE0
+                             : (mode == 1 ? (ahead ? FIN_FORCE1_AHEAD : FIN_FORCE1) : (ahead ? FIN_FORCE2_AHEAD : FIN_FORCE2));
     fin(reps, nreps, op, sysmask, 0.0, s);
     return check(hipGetLastError());
 }
@@ -2665,10 +2669,10 @@ int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s)
     return 0;  // the drift publishes its maximum with atomics; nothing to finish
 }
 
-int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, bool ahead)
 {
     if (ntiles <= 0) return 0;
-    hipLaunchKernelGGL(k_drift_brownian, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps);
+    hipLaunchKernelGGL(k_drift_brownian, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, ahead ? 1 : 0);
     return check(hipGetLastError());
 }
 
