@@ -5,22 +5,32 @@ spheres, N = 10^6, rho = 1.02, skin 0.3, Newtonian NVE (velocity Verlet,
 dt = 0.005) from a seeded jittered simple-cubic lattice, relaxed on the
 device by capped steepest descent and thermalised at kBT = 0.2 by velocity
 rescaling (all untimed).  Inputs are resident in HBM when the timed region
-starts.  The timed region is K steps of do_time_step (neighbour rebuilds on
-the skin trigger included) in one sc_do_time_steps call, bracketed by device
-synchronisation.  `value` is the FP64 drop-in form; the mixed-precision form
-(FP32 pair arithmetic) runs from the same state and is reported beside it
-with its force error and NVE drift (linear fit of the total energy over a
-separate run, scaled to 10^5 steps).
+starts.
+
+Each precision form (FP64, mixed) starts from the same thermalised state:
+update_neighbor_lists!, W untimed warm-up steps, then the K timed steps in one
+sc_do_time_steps call with per-kernel timing off (device time from HIP
+events around the call; the call synchronises).  `value` is the FP64 form's
+rate over those K steps; each form also reports its rebuild count in the K
+steps, a separate run with per-kernel timing on (force and drift launch
+averages), its steady step without rebuilds, and a long window of
+--long-steps steps (>= 5 rebuild intervals) whose ratio is the like-for-like
+mixed / FP64 speed-up; the mixed form's force error on the common state and
+both forms' NVE drift (linear fit of the total energy over a separate run,
+scaled to 10^5 steps).
 
 roofline: the dominant kernel is the pair-force kernel (fused with the
 second half-kick and the COM partial sums).  Algorithmic bytes per particle:
 r (24) + D (8) + v read/write (48) + F write (24) + 4 (n_half+1) neighbour
-index bytes, n_half measured on the device lists; its average duration comes
-from HIP events on the context stream over the timed region.
+index bytes, n_half measured on the device lists; its average launch time
+comes from HIP events on the context stream in the timed-kernel run.
+
+batched / served: bounded runs of the replica workloads (extra keys), see
+batched_summary and simulationcode_amd/serve.summary.
 
 cpu_baseline: the oracle (FP64 C restatement of the reference's serial
-algorithm, one thread) on the same N = 10^6 state, timed for a bounded number
-of steps (~15 s), rebuilds included.
+algorithm) on the same N = 10^6 state on the host: 10 warm-up steps, then the
+median of three runs, once on one thread and once on all host cores.
 
 --workload batched: 256 independent replicas (N log-uniform in [500, 20000],
 75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian, per-replica
@@ -81,18 +91,6 @@ def total_energy(ctx):
     return ctx.full_energy() + ctx.kinetic_energy()
 
 
-def timed_run(ctx, steps, warmup):
-    ctx.do_time_steps(warmup)
-    reb0 = ctx.rebuild_count()
-    ctx.set_kernel_timing(True)
-    t0 = time.perf_counter()
-    ctx.do_time_steps(steps)  # synchronises on return
-    wall = time.perf_counter() - t0
-    kt = ctx.kernel_times()
-    ctx.set_kernel_timing(False)
-    return {"wall_s": wall, "device_ms": ctx.last_elapsed_ms(), "rebuilds": ctx.rebuild_count() - reb0, "kt": kt}
-
-
 def drift_run(ctx, steps, samples=40):
     """Total-energy drift: least-squares slope of E over `steps` steps,
     relative to |E|, scaled to 10^5 steps; also the relative spread."""
@@ -106,7 +104,61 @@ def drift_run(ctx, steps, samples=40):
     return float(slope * 1e5 / abs(e[0])), float((max(e) - min(e)) / abs(e[0]))
 
 
+def leg(ctx, state, args):
+    """One precision form from the common state: the forced rebuild of
+    update_neighbor_lists!, W untimed warm-up steps, then the K headline steps
+    untimed (device time of the call from HIP events on the context stream).
+    Then, from where that run ends: a run with per-kernel timing on (force and
+    drift launch averages), the steady step without rebuilds (the shortest of
+    three 8-step calls started right after a rebuild) and a long window of at
+    least five rebuild intervals."""
+    ctx.set_state(r=state["r"], v=state["v"], F=state["F"])
+    ctx.update_neighbor_lists()
+    ctx.do_time_steps(args.warmup)
+    reb0 = ctx.rebuild_count()
+    t0 = time.perf_counter()
+    ctx.do_time_steps(args.steps)  # synchronises on return
+    wall = time.perf_counter() - t0
+    ms = ctx.last_elapsed_ms()
+    out = {"ms_per_step": ms / args.steps, "rebuilds": ctx.rebuild_count() - reb0, "wall_s": wall}
+    ctx.set_kernel_timing(True)
+    nt = max(args.steps, 200)
+    ctx.do_time_steps(nt)
+    kt = ctx.kernel_times()
+    ctx.set_kernel_timing(False)
+    out["force_kernel_avg_us"] = kt["force_ms"] / max(kt["force_launches"], 1) * 1e3
+    out["drift_kernel_avg_us"] = kt["drift_ms"] / max(kt["drift_launches"], 1) * 1e3
+    out["timed_run_launches"] = {"steps": nt, "force": kt["force_launches"], "drift": kt["drift_launches"]}
+    steady = []
+    for _ in range(3):
+        ctx.update_neighbor_lists()
+        r0 = ctx.rebuild_count()
+        ctx.do_time_steps(8)
+        if ctx.rebuild_count() == r0:
+            steady.append(ctx.last_elapsed_ms() / 8)
+    out["steady_ms_per_step"] = min(steady) if steady else None
+    r0 = ctx.rebuild_count()
+    ctx.do_time_steps(args.long_steps)
+    out["long_window"] = {"steps": args.long_steps, "ms_per_step": ctx.last_elapsed_ms() / args.long_steps,
+                          "rebuilds": ctx.rebuild_count() - r0}
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(op_state, budget_s=15.0):
+    """The oracle on the host, after the protocol of 10 untimed warm-up steps:
+    the median rate of three runs of equal length (sized from the warm-up to
+    about budget_s / 3 each), rebuilds included; once on one thread (the
+    serial C restatement) and once on all host cores (OpenMP)."""
     import oracle
     r, v, F, D, L, N = op_state
     op = oracle.make_params(dims=3, N=N, box_size=L, force_cutoff=1.25, skin=0.3, D=D,
@@ -114,31 +166,37 @@ def cpu_baseline(op_state, budget_s=15.0):
                             maxneighbours=128)
     st = oracle.State(op, r, v, D, F=F)
     st.update_neighbor_lists(cells=True)
-    steps = 0
     t0 = time.perf_counter()
-    while True:
-        st.do_time_steps(1, cells=True)
-        steps += 1
-        el = time.perf_counter() - t0
-        if (el > budget_s and steps >= 3) or steps >= 2000:
-            break
-    out = {"value": N * steps / el, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-           "sample": f"oracle (serial FP64 C restatement, cell-list rebuild) on the same N={N} state, "
-                     f"{steps} steps in {el:.1f} s incl. {int(st.rebuilds[0]) - 1} rebuilds"}
-    # the same step on all host cores (OpenMP: per-particle rows, gathered
-    # forces); a 3-step pilot sizes one run of about budget_s
+    st.do_time_steps(10, cells=True)
+    per = (time.perf_counter() - t0) / 10
+    n = int(min(max(budget_s / 3 / max(per, 1e-6), 3), 2000))
+    rates, reb = [], []
+    for _ in range(3):
+        b0 = int(st.rebuilds[0])
+        t0 = time.perf_counter()
+        st.do_time_steps(n, cells=True)
+        rates.append(N * n / (time.perf_counter() - t0))
+        reb.append(int(st.rebuilds[0]) - b0)
+    model = cpu_model()
+    out = {"value": float(np.median(rates)), "unit": "particle-steps/s", "cores": 1, "kind": "port",
+           "cpu": model, "protocol": "10 warm-up steps, median of 3 runs",
+           "sample": f"oracle (serial FP64 C restatement, cell-list rebuild) on the same N={N} state: "
+                     f"3 runs of {n} steps ({reb} rebuilds), rates {[round(x) for x in rates]}"}
     threads = min(16, len(os.sched_getaffinity(0)))
     rb, vb, Fb = (np.ascontiguousarray(x, dtype=np.float64).copy() for x in (r, v, F))
     t0 = time.perf_counter()
-    oracle.baseline_steps(op, 3, threads, rb, vb, Fb, D)
-    per = (time.perf_counter() - t0) / 3
-    n = int(min(max(budget_s / max(per, 1e-6), 5), 5000))
-    t0 = time.perf_counter()
-    nreb = oracle.baseline_steps(op, n, threads, rb, vb, Fb, D)
-    el = time.perf_counter() - t0
-    out["all_cores"] = {"value": N * n / el, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+    oracle.baseline_steps(op, 10, threads, rb, vb, Fb, D)
+    per = (time.perf_counter() - t0) / 10
+    n = int(min(max(budget_s / 3 / max(per, 1e-6), 3), 5000))
+    rates = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.baseline_steps(op, n, threads, rb, vb, Fb, D)
+        rates.append(N * n / (time.perf_counter() - t0))
+    out["all_cores"] = {"value": float(np.median(rates)), "unit": "particle-steps/s", "cores": threads,
+                        "kind": "port", "cpu": model, "protocol": "10 warm-up steps, median of 3 runs",
                         "sample": f"the same Newtonian step on {threads} OpenMP threads (per-particle rows, "
-                                  f"gathered forces), {n} steps in {el:.1f} s incl. {nreb + 1} rebuilds"}
+                                  f"gathered forces): 3 runs of {n} steps, rates {[round(x) for x in rates]}"}
     return out
 
 
@@ -170,13 +228,13 @@ def bench_large(args, nat, synth):
     tot, mx = ctx.neighbor_stats()
     n_half = tot / 2.0 / N
 
-    res64 = timed_run(ctx, args.steps, args.warmup)
-    ms64 = res64["device_ms"] / args.steps
-    log(f"[bench] fp64: {ms64 * 1e3:.1f} us/step, rebuilds {res64['rebuilds']}")
-    value = N * args.steps / (res64["device_ms"] / 1e3)
+    res64 = leg(ctx, state, args)
+    ms64 = res64["ms_per_step"]
+    log(f"[bench] fp64: {ms64 * 1e3:.1f} us/step ({res64['rebuilds']} rebuilds), steady "
+        f"{res64['steady_ms_per_step'] * 1e3:.1f} us, long window {res64['long_window']['ms_per_step'] * 1e3:.1f} us")
+    value = N / (ms64 / 1e3)
 
-    kt = res64["kt"]
-    f_avg_s = kt["force_ms"] / max(kt["force_launches"], 1) / 1e3
+    f_avg_s = res64["force_kernel_avg_us"] / 1e6
     bytes_per_particle = 24 + 8 + 48 + 24 + 4 * (n_half + 1)
     achieved = N * bytes_per_particle / f_avg_s / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -184,11 +242,14 @@ def bench_large(args, nat, synth):
                 "traffic": measured_traffic("traffic_k_force1_berthier3d_N1e6_fp64"),
                 "kernel": "k_force<1> (pair forces + second half-kick + COM partials)",
                 "kernel_avg_us": round(f_avg_s * 1e6, 2), "bytes_per_particle": round(bytes_per_particle, 2),
-                "n_half_mean": round(n_half, 3),
-                "drift_kernel_avg_us": round(kt["drift_ms"] / max(kt["drift_launches"], 1) * 1e3, 2)}
+                "n_half_mean": round(n_half, 3), "drift_kernel_avg_us": round(res64["drift_kernel_avg_us"], 2)}
     if cc:
         roofline.update({"copy_ceiling_GBs": round(cc, 1), "frac_of_copy_ceiling": round(achieved / cc, 4)})
         log(f"[bench] measured copy ceiling {cc:.0f} GB/s")
+    # whole-step bytes: the force kernel's plus the drift kernel's 184 B and the
+    # rebuild's amortised share (8d+8 + 4 n_half + 8 per particle per rebuild)
+    lw = res64["long_window"]
+    step_bytes = bytes_per_particle + 184 + (32 + 4 * n_half + 8) * lw["rebuilds"] / lw["steps"]
     out = {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms64, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64",
@@ -196,7 +257,9 @@ def bench_large(args, nat, synth):
            "config": {"workload": f"3D polydisperse non-additive soft spheres, N={N}, rho=1.02, Newtonian NVE "
                                   f"dt=0.005, skin 0.3, kBT=0.2 start", "N": N, "dims": 3,
                       "potential": "Berthier (rc=1.25, zeta=0.2)", "dynamics": "Newtonian", "dt": 0.005},
-           "roofline": roofline, "fp64": {"rebuilds": res64["rebuilds"], "wall_s": res64["wall_s"]}}
+           "roofline": roofline,
+           "fp64": dict(res64, step_roofline_frac=round(N * step_bytes / (lw["ms_per_step"] / 1e3) / 1e9
+                                                        / HBM_PEAK_GBS, 4))}
 
     if args.drift_steps > 0:
         ctx.set_state(r=state["r"], v=state["v"], F=state["F"])
@@ -218,33 +281,102 @@ def bench_large(args, nat, synth):
         fn = np.linalg.norm(F64, axis=1)
         dn = np.linalg.norm(F32 - F64, axis=1)
         ctx.close()
-        ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
-        ctxm.update_neighbor_lists()
-        resm = timed_run(ctxm, args.steps, args.warmup)
-        msm = resm["device_ms"] / args.steps
-        kmt = resm["kt"]
-        out["mixed"] = {"value": N * args.steps / (resm["device_ms"] / 1e3), "ms_per_step": msm,
-                        "speedup_vs_fp64": ms64 / msm,
-                        "max_rel_force_err_vs_mean": float(dn.max() / fn.mean()),
-                        "max_rel_force_err_vs_max": float(dn.max() / fn.max()),
-                        "rebuilds": resm["rebuilds"],
-                        "force_kernel_avg_us": kmt["force_ms"] / max(kmt["force_launches"], 1) * 1e3}
+        resm = leg(ctxm, state, args)
+        msm = resm["ms_per_step"]
+        out["mixed"] = dict(resm, value=N / (msm / 1e3), speedup_vs_fp64=ms64 / msm,
+                            speedup_vs_fp64_steady=res64["steady_ms_per_step"] / resm["steady_ms_per_step"]
+                            if resm["steady_ms_per_step"] else None,
+                            speedup_vs_fp64_long_window=lw["ms_per_step"] / resm["long_window"]["ms_per_step"],
+                            max_rel_force_err_vs_mean=float(dn.max() / fn.mean()),
+                            max_rel_force_err_vs_max=float(dn.max() / fn.max()))
         if args.drift_steps > 0:
             ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
             ctxm.update_neighbor_lists()
             dm, flm = drift_run(ctxm, args.drift_steps)
             out["mixed"].update({"nve_rel_drift_per_1e5_steps": dm, "nve_rel_fluct": flm})
-        log(f"[bench] mixed: {msm * 1e3:.1f} us/step")
+        log(f"[bench] mixed: {msm * 1e3:.1f} us/step ({resm['rebuilds']} rebuilds), steady "
+            f"{resm['steady_ms_per_step'] * 1e3:.1f} us, long window {resm['long_window']['ms_per_step'] * 1e3:.1f} us, "
+            f"force err {out['mixed']['max_rel_force_err_vs_mean']:.3g} of mean |F|")
         ctxm.close()
     else:
         ctx.close()
 
+    if not args.no_batched:
+        out["batched"] = batched_summary(args, nat)
+    if not args.no_served:
+        from simulationcode_amd import serve
+        out["served"] = serve.summary(args, log)
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 thread and all host cores)")
         out["cpu_baseline"] = cpu_baseline((state["r"], state["v"], state["F"], D, L, N))
     else:
         out["cpu_baseline"] = None
     return out
+
+
+def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300, 0.005, 0.02), retune=False):
+    """Admit `specs` (cell divisor cd, build launch shape) into `streams`
+    batches, relax, warm up, optionally re-tune the launch shape on the full
+    batch, then time `steps` steps.  Returns the measurements."""
+    if streams > 1:
+        b = nat.BatchGroup(len(specs) + 1, streams=streams, graph=graph)
+    else:
+        b = nat.Batch(len(specs) + 1, graph=graph)
+    b.set_build_launch(*shape)
+    ids, Ns, dims = [], [], []
+    t0 = time.perf_counter()
+    for s in specs:
+        p, D, r, v = W_replica(s, cd)
+        ids.append(b.add(p, D, r, v))
+        Ns.append(s["N"])
+        dims.append(s["dims"])
+    t_admit = time.perf_counter() - t0
+    b.minimize(*relax)
+    b.step(warmup)
+    full_tune = None
+    if retune:
+        tab, best = b.autotune(100)
+        shape = (tab[best]["block_size"], tab[best]["lanes_per_particle"])
+        full_tune = [dict(e, ms_per_step=round(e["ms_per_step"], 5)) for e in tab]
+    reb0 = sum(b.stats(i)["rebuilds"] for i in ids)
+    b.step(steps)
+    ms = b.info()["last_ms"]
+    reb = sum(b.stats(i)["rebuilds"] for i in ids) - reb0
+    tot = sum(Ns)
+    # algorithmic bytes per particle-step, Brownian with on-device noise:
+    # r 2*8d + F write 8d + disp 2*8d + D 8 + 4 (n_half + 1)
+    byt = 0.0
+    for rid, n, d in zip(ids, Ns, dims):
+        t, _ = b.neighbor_stats(rid)
+        byt += n * (40 * d + 8 + 4 * (t / 2.0 / n + 1))
+    achieved = byt * steps / (ms / 1e3) / 1e9
+    b.close() if hasattr(b, "close") else None
+    return {"value": tot * steps / (ms / 1e3), "ms_per_step": ms / steps, "replicas": len(ids), "particles": tot,
+            "rebuilds_per_step": reb / steps, "admit_s": round(t_admit, 2), "shape": shape,
+            "autotune_full_batch": full_tune,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                         "kernel": "whole batched step (all kernels)",
+                         "bytes_per_particle_step_mean": round(byt / tot, 2)}}
+
+
+def W_replica(spec, cd):
+    from simulationcode_amd import workloads as W
+    return W.replica(**dict(spec, cell_divisor=cd))
+
+
+def batched_summary(args, nat, steps=300, warmup=20):
+    """The batched workload at a bounded length for the default run: the 256
+    replicas at the launch choices the full autotuned run settles on (cell
+    divisor 1, build 256 threads x 16 lanes per slot, two streams)."""
+    from simulationcode_amd import workloads as W
+    t0 = time.perf_counter()
+    res = run_batched(nat, W.batched_specs(args.replicas), 1, (256, 16), 2, False, warmup, steps)
+    res.update({"steps": steps, "warmup": warmup, "dt": 1e-4, "dynamics": "Brownian", "dtype": "mixed",
+                "cell_divisor": 1, "streams": 2, "wall_s": round(time.perf_counter() - t0, 1)})
+    log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step, {res['value']:.3g} particle-steps/s, "
+        f"roofline {res['roofline']['frac']:.3f}")
+    return res
 
 
 def bench_batched(args, nat):
@@ -258,55 +390,23 @@ def bench_batched(args, nat):
         cd, shape, tune = W.autotune_batch(specs, pilot=64, steps=60)
     log(f"[bench] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
         f"({time.perf_counter() - t0:.1f}s)")
-    if args.streams > 1:
-        b = nat.BatchGroup(len(specs) + 1, streams=args.streams, graph=args.graph)
-    else:
-        b = nat.Batch(len(specs) + 1, graph=args.graph)
-    b.set_build_launch(*shape)
-    ids, Ns, dims = [], [], []
-    t0 = time.perf_counter()
-    for s in specs:
-        p, D, r, v = W.replica(**dict(s, cell_divisor=cd))
-        ids.append(b.add(p, D, r, v))
-        Ns.append(s["N"])
-        dims.append(s["dims"])
-    log(f"[bench] admitted {len(ids)} replicas, {sum(Ns)} particles ({time.perf_counter() - t0:.1f}s)")
-    b.minimize(300, 0.005, 0.02)
-    b.step(args.warmup)
-    full_tune = None
-    if not args.cell_divisor:
-        # the launch shape changes no result: re-tune it on the full batch
-        # (the pilot holds a quarter of the replicas, so fewer rebuild per step)
-        tab, best = b.autotune(100)
-        shape = (tab[best]["block_size"], tab[best]["lanes_per_particle"])
-        full_tune = [dict(e, ms_per_step=round(e["ms_per_step"], 5)) for e in tab]
-        log(f"[bench] full-batch shape {shape[0]} x {shape[1]} lanes/slot")
-    b.step(args.steps)
-    ms = b.info()["last_ms"]
-    tot = sum(Ns)
-    value = tot * args.steps / (ms / 1e3)
-    # algorithmic bytes per particle-step, Brownian with on-device noise:
-    # r 2*8d + F write 8d + disp 2*8d + D 8 + 4 (n_half + 1)
-    byt = 0.0
-    for rid, n, d in zip(ids, Ns, dims):
-        t, _ = b.neighbor_stats(rid)
-        byt += n * (40 * d + 8 + 4 * (t / 2.0 / n + 1))
-    achieved = byt * args.steps / (ms / 1e3) / 1e9
-    log(f"[bench] batched: {ms / args.steps * 1e3:.1f} us/step over {tot} particles")
-    return {"metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True, "scaling": "weak",
+    # the launch shape changes no result: with the autotuner on, it is re-tuned
+    # on the full batch (the pilot holds a quarter of the replicas, so fewer
+    # rebuild per step)
+    res = run_batched(nat, specs, cd, shape, args.streams, args.graph, args.warmup, args.steps,
+                      retune=not args.cell_divisor)
+    log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step over {res['particles']} particles")
+    return {"metric": METRIC, "value": res["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "mixed (f32 pair / f64 state)",
             "data": "synthetic (seeded lattices relaxed on device)",
-            "config": {"workload": f"{len(ids)} independent polydisperse replicas, N log-uniform [500, 20000], "
+            "config": {"workload": f"{res['replicas']} independent polydisperse replicas, N log-uniform [500, 20000], "
                                    "75% 3D / 25% 2D, Brownian dt=1e-4, kBT U[0.1,0.3], per-replica seeds",
-                       "replicas": len(ids), "particles": tot, "graph": args.graph, "streams": args.streams,
-                       "cell_divisor": cd, "build_block": shape[0], "build_lanes_per_slot": shape[1],
-                       "autotune": tune, "autotune_full_batch": full_tune},
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                         "kernel": "whole batched step (all kernels)",
-                         "bytes_per_particle_step_mean": round(byt / tot, 2)},
-            "cpu_baseline": None}
+                       "replicas": res["replicas"], "particles": res["particles"], "graph": args.graph,
+                       "streams": args.streams, "cell_divisor": cd, "build_block": res["shape"][0],
+                       "build_lanes_per_slot": res["shape"][1], "autotune": tune,
+                       "autotune_full_batch": res["autotune_full_batch"]},
+            "rebuilds_per_step": res["rebuilds_per_step"], "roofline": res["roofline"], "cpu_baseline": None}
 
 
 def main():
@@ -320,6 +420,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-copy-ceiling", action="store_true", help="large: skip the measured device-copy ceiling")
+    ap.add_argument("--no-batched", action="store_true", help="large: skip the bounded batched summary")
+    ap.add_argument("--no-served", action="store_true", help="large: skip the bounded served summary")
+    ap.add_argument("--long-steps", type=int, default=1000,
+                    help="large: steps of each form's long window (at least five rebuild intervals)")
     ap.add_argument("--graph", action="store_true",
                     help="batched: replay the step as a captured hipGraph (eager launches measured faster)")
     ap.add_argument("--streams", type=int, default=2,

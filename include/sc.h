@@ -88,16 +88,20 @@ int sc_get_state(sc_ctx *ctx, double *r, double *v, double *F, double *disp,
                  double *disp2);
 
 /* update_neighbor_lists! -- src/NeighborLists.jl:24-53 (counts a rebuild,
- * zeroes the displacements, rebuilds the Verlet lists on the device). */
+ * zeroes the displacements, rebuilds the Verlet lists on the device).  It
+ * also clears an error a previous call left on the device (see
+ * sc_do_time_steps): the rebuild checks the lists again. */
 int sc_update_neighbor_lists(sc_ctx *ctx);
 
 /* Reference-layout lists for parity and for the CPU Monte-Carlo path
  * (NeighborListStruct fields, src/DataStructs.jl:248-251): counts of length
- * N, lists column-major maxneigh x N, 1-based indices.  Full rows hold every
- * j with d2 <= R_v^2; half rows hold each pair with
- * d2 < (sqrt(force_cutoff2)*sigma_ij + skin)^2 once (in the row of the smaller
- * index).  Entry order within a row is unspecified, as in the reference.
- * Any output may be NULL. */
+ * N, lists column-major maxneigh x N, 1-based indices.  As in the reference,
+ * both lists are the ones the last rebuild stored (src/NeighborLists.jl:82-104),
+ * evaluated at the positions of that rebuild, whatever the particles did
+ * since: full rows hold every j with d2 <= R_v^2; half rows hold each pair
+ * with d2 < (sqrt(force_cutoff2)*sigma_ij + skin)^2 once (in the row of the
+ * smaller index).  Entry order within a row is unspecified, as in the
+ * reference.  Any output may be NULL. */
 int sc_get_neighbor_lists(sc_ctx *ctx, int64_t maxneigh, int64_t *nfull, int64_t *full,
                           int64_t *nhalf, int64_t *half);
 
@@ -110,7 +114,10 @@ int sc_calculate_new_forces(sc_ctx *ctx, const double *noise);
  * velocity Verlet with COM removal, optional rescale, skin-triggered
  * rebuild) and the intended Brownian step of src/MD.jl:429-471.  noise:
  * nsteps consecutive N x dims AoS blocks, or NULL for on-device Philox.
- * The state is reference-consistent when the call returns. */
+ * The state is reference-consistent when the call returns.  A device error
+ * (overflow, non-finite position) stops the call with its status and stays
+ * recorded: this call, sc_calculate_new_forces and the minimisers then refuse
+ * with the same status until sc_update_neighbor_lists has run. */
 int sc_do_time_steps(sc_ctx *ctx, int64_t nsteps, const double *noise);
 
 /* calculate_full_energy -- src/AuxiliaryFunctions.jl:22-34 (+ find_energy

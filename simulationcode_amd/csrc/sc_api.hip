@@ -30,7 +30,6 @@ struct sc_ctx {
     std::string err;
     double *d_noise = nullptr;
     size_t noise_cap = 0;
-    long long host_step = 0;
     double last_ms = 0.0;
     std::vector<void *> allocs;
     bool timing = false;
@@ -86,18 +85,36 @@ int push_rep(sc_ctx *c)
     return SC_OK;
 }
 
+// Device error bits as a status.
+int err_status(sc_ctx *c, int bits, const char *when)
+{
+    if (bits & SC_EBIT_OVERFLOW)
+        return fail(c, SC_ERR_OVERFLOW,
+                    std::string(when) + "neighbour list overflow: a particle has more than maxneighbours=" +
+                        std::to_string(c->h.cap) + " neighbours within the Verlet radius");
+    if (bits & SC_EBIT_NONFINITE) return fail(c, SC_ERR_NONFINITE, std::string(when) + "non-finite particle position");
+    if (bits & SC_EBIT_NOISE)
+        return fail(c, SC_ERR_STATE, std::string(when) + "a force evaluation had no injected noise block for its step");
+    return SC_OK;
+}
+
 // Reads the device error bits and turns them into a status.
 int check_dyn(sc_ctx *c)
 {
     RepDyn d;
     HIPCHK(c, hipMemcpyAsync(&d, c->h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, c->s));
     HIPCHK(c, hipStreamSynchronize(c->s));
-    if (d.err & SC_EBIT_OVERFLOW)
-        return fail(c, SC_ERR_OVERFLOW,
-                    "neighbour list overflow: a particle has more than maxneighbours=" +
-                        std::to_string(c->h.cap) + " neighbours within the Verlet radius");
-    if (d.err & SC_EBIT_NONFINITE) return fail(c, SC_ERR_NONFINITE, "non-finite particle position");
-    return SC_OK;
+    return err_status(c, d.err, "");
+}
+
+// A context whose device state carries an error from an earlier call is not
+// stepped again (its positions or lists are not valid) until
+// sc_update_neighbor_lists, which clears the bits, has run -- typically after
+// sc_set_state has uploaded a valid state.
+int refuse_if_failed(sc_ctx *c, const RepDyn &d)
+{
+    if (!d.err) return SC_OK;
+    return err_status(c, d.err, "an earlier call left this error (rebuild with sc_update_neighbor_lists to continue): ");
 }
 
 int read_dyn(sc_ctx *c, RepDyn *d)
@@ -124,7 +141,9 @@ int download_field(sc_ctx *c, double *dst, int field)
     return SC_OK;
 }
 
-int set_noise(sc_ctx *c, const double *noise, int64_t blocks)
+// Injected noise for the next `blocks` force evaluations, the first of which
+// is the device's evaluation number `step0` (RepDyn::step).
+int set_noise(sc_ctx *c, const double *noise, int64_t blocks, long long step0)
 {
     if (!noise) {
         if (c->h.noise != nullptr) {
@@ -148,7 +167,8 @@ int set_noise(sc_ctx *c, const double *noise, int64_t blocks)
     }
     HIPCHK(c, hipMemcpyAsync(c->d_noise, noise, need * sizeof(double), hipMemcpyHostToDevice, c->s));
     c->h.noise = c->d_noise;
-    c->h.noise_step0 = c->host_step;
+    c->h.noise_step0 = step0;
+    c->h.noise_blocks = blocks;
     return push_rep(c);
 }
 
@@ -444,11 +464,10 @@ int sc_calculate_new_forces(sc_ctx *c, const double *noise)
     int rc = read_dyn(c, &d);
     if (rc) return rc;
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
-    if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, 1))) return rc;
+    if ((rc = refuse_if_failed(c, d))) return rc;
+    if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, 1, d.step))) return rc;
     KCHK(c, sc::launch_force(c->d_tiles, c->ntiles, c->d_rep, 1, 0, 7, c->s));
-    c->host_step += 1;
-    HIPCHK(c, hipStreamSynchronize(c->s));
-    return SC_OK;
+    return check_dyn(c);
 }
 
 int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
@@ -458,7 +477,8 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     int rc = read_dyn(c, &d);
     if (rc) return rc;
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
-    if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, nsteps))) return rc;
+    if ((rc = refuse_if_failed(c, d))) return rc;
+    if ((rc = set_noise(c, c->h.sys != SC_SYS_NEWTONIAN ? noise : nullptr, nsteps, d.step))) return rc;
     const int nt = c->ntiles;
     const bool timed = c->timing;
     if (timed && (rc = ensure_events(c, 4 * (size_t)nsteps))) return rc;
@@ -504,7 +524,6 @@ int sc_do_time_steps(sc_ctx *c, int64_t nsteps, const double *noise)
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    c->host_step += nsteps;
     if (timed) {
         // Newtonian/Langevin: [drift][rebuild][force]; Brownian: [force][drift][rebuild]
         for (size_t q = 0; q < ev; q += 3) {
@@ -653,6 +672,7 @@ int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *ma
     int rc = read_dyn(c, &d);
     if (rc) return rc;
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    if ((rc = refuse_if_failed(c, d))) return rc;
     const int nt = c->ntiles;
     // pair forces only: a Newtonian view of the replica for the force kernel
     Rep saved = c->h;
@@ -664,7 +684,6 @@ int sc_minimize(sc_ctx *c, int64_t iters, double dt, double max_step, double *ma
         KCHK(c, sc::launch_min_move(c->d_tiles, nt, c->d_rep, 1, dt, c->s));
         KCHK(c, sc::launch_rebuild(c->d_tiles, nt, c->d_rep, 1, c->geom, c->s));
     }
-    c->host_step += iters;
     c->h = saved;
     if ((rc = push_rep(c))) return rc;
     if ((rc = read_dyn(c, &d))) return rc;
@@ -693,6 +712,7 @@ int sc_minimize_until(sc_ctx *c, int64_t max_iters, double dt, double max_step, 
     int rc = read_dyn(c, &d);
     if (rc) return rc;
     if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    if ((rc = refuse_if_failed(c, d))) return rc;
     // Each iteration checks max(max|F|, step) <= tolerance on the device after
     // its move (FIN_MINSCALE); from then on the force, scale and move kernels
     // of the remaining launched iterations return at once.  The host launches
@@ -721,7 +741,6 @@ int sc_minimize_until(sc_ctx *c, int64_t max_iters, double dt, double max_step, 
     if (rc) return rc;
     if (rc2) return rc2;
     if ((rc = read_dyn(c, &d))) return rc;
-    c->host_step += d.min_iters;  // force evaluations done (the device's step counter)
     if (iterations) *iterations = d.min_iters;
     if (maxforce) *maxforce = d.result[3];
     if (converged) *converged = conv ? 1 : 0;

@@ -270,9 +270,21 @@ int sc_batch_add(sc_batch *b, const sc_params *p, const double *D, const double 
     int one = 1;
     BCHK(b, hipMemcpyAsync(&h.dyn->rebuild, &one, sizeof(int), hipMemcpyHostToDevice, b->s));
     BK(b, sc::launch_rebuild(b->d_tiles, b->ntiles, b->d_reps, b->nact, b->geom, b->s, true));
+    RepDyn d;
+    BCHK(b, hipMemcpyAsync(&d, h.dyn, sizeof(RepDyn), hipMemcpyDeviceToHost, b->s));
     BCHK(b, hipStreamSynchronize(b->s));
+    // only this replica's state decides the admission (the others' errors
+    // surface at their next step); a replica that fails its first rebuild is
+    // not admitted and its slot is freed again
+    if (d.err) {
+        const int code = (d.err & SC_EBIT_OVERFLOW) ? SC_ERR_OVERFLOW : SC_ERR_NONFINITE;
+        int rc2 = sc_batch_remove(b, slot);
+        if (rc2) return rc2;
+        return bfail(b, code, code == SC_ERR_OVERFLOW ? "admission: neighbour list overflow (maxneighbours)"
+                                                      : "admission: non-finite state");
+    }
     *id = slot;
-    return batch_errors(b);
+    return SC_OK;
 }
 
 int sc_batch_remove(sc_batch *b, int32_t id)

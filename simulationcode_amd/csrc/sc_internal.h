@@ -38,6 +38,7 @@ enum { BUILD_SLOT = 0, BUILD_CELLS = 1, BUILD_WIN = 2 };
 // error bits raised on the device
 #define SC_EBIT_OVERFLOW 1
 #define SC_EBIT_NONFINITE 2
+#define SC_EBIT_NOISE 4  // a force evaluation found no injected noise block for its step
 
 // last-block-done counters
 enum { CTR_FORCE = 0, CTR_DRIFT, CTR_BUILD, CTR_ENERGY, CTR_EKIN, CTR_MISC, CTR_EKCOM, CTR_N };
@@ -93,9 +94,11 @@ struct Rep {
     double kBT, gamma, dt, m;
     double min_tol;      // minimiser: stop once max |F| <= min_tol (< 0: never)
     uint64_t seed;
-    // ---- injected noise (NULL => Philox): block s at noise + (step - noise_step0) * n * dims
+    // ---- injected noise (NULL => Philox): block s at noise + (step - noise_step0) * n * dims,
+    //      for 0 <= step - noise_step0 < noise_blocks
     const double *noise;
     long long noise_step0;
+    long long noise_blocks;
     // ---- device buffers
     double4 *pos, *pos_alt;
     double *v, *v_alt, *F, *disp;
@@ -127,7 +130,6 @@ int launch_force(const Tile *tiles, int ntiles, const Rep *reps, int nreps, int 
                  bool finish = true, bool ahead = false);
 int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipStream_t s, bool ahead = false);
 int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s, bool ahead = false);
-int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s);
 int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, bool ahead = false);
 int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 struct RebuildGeom {

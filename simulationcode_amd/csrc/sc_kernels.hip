@@ -612,10 +612,19 @@ __device__ void force_tile(const Rep &R, const Tile &t)
         if (R.sys != 0) {
             double xi[3];
             if (R.noise) {
-                const SC_AS1 double *nz = gp(R.noise) + (size_t)(step - R.noise_step0) * (size_t)n * DIMS +
-                                   (size_t)gp(R.orig)[i] * DIMS;
+                // injected blocks cover noise_step0 .. noise_step0 + noise_blocks - 1;
+                // outside them (a call sequence the host did not plan) no block is read
+                const long long blk = step - R.noise_step0;
+                if (blk >= 0 && blk < R.noise_blocks) {
+                    const SC_AS1 double *nz = gp(R.noise) + (size_t)blk * (size_t)n * DIMS +
+                                              (size_t)gp(R.orig)[i] * DIMS;
 #pragma unroll
-                for (int c = 0; c < DIMS; c++) xi[c] = nz[c];
+                    for (int c = 0; c < DIMS; c++) xi[c] = nz[c];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < DIMS; c++) xi[c] = 0.0;
+                    gatomic_or(&dyn->err, SC_EBIT_NOISE);
+                }
             } else if (PREC == 1) {
                 float z[4];
                 normals4_f32((uint32_t)gp(R.orig)[i], step, R.seed, z);
@@ -740,10 +749,16 @@ __global__ __launch_bounds__(SC_TB) void k_finalize(const Tile *tiles, const Rep
 // Every kernel returns at once for replicas whose rebuild flag is clear.
 // ====================================================================
 
+// A rebuild requested between steps (F stays live).  It also clears the
+// replica's error bits: the rebuild validates the lists again (overflow) and
+// the next drift the positions (non-finite).
 __global__ void k_request_rebuild(Rep *reps, int nreps)
 {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < nreps) gp(reps[r].dyn)->rebuild = 1 | REBUILD_KEEP_F;  // between steps: F stays live
+    if (r < nreps) {
+        gp(reps[r].dyn)->rebuild = 1 | REBUILD_KEEP_F;
+        gp(reps[r].dyn)->err = 0;
+    }
 }
 
 // Error bits of every replica into one array (one host copy per check).
@@ -1904,14 +1919,18 @@ __global__ __launch_bounds__(SC_TB, 3) void k_build_win(const Rep *reps, const i
     }
 }
 
-// Full rows at the current positions over the rebuild-time cells, for the
-// reference-layout list export.
+// Full rows as the last rebuild stored them (src/NeighborLists.jl:82-104:
+// the reference fills its full list at rebuild time and leaves it until the
+// next rebuild), for the reference-layout list export.  pos_alt is written
+// only by the rebuild's permutation, so between rebuilds it holds the
+// rebuild-time positions in the current slot order; each slot's cell is the
+// cell it was binned into.
 template <int DIMS>
 __device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *counts, int cap)
 {
     const int n = R.n, k = t.first + threadIdx.x;
     if (k >= n) return;
-    const double4 p = ld4(gp(R.pos) + (k));
+    const double4 p = ld4(gp(R.pos_alt) + (k));
     int cc[3];
     cell_of_pos<DIMS>(R, p, cc);
     const int s = R.srange;
@@ -1925,7 +1944,7 @@ __device__ void full_rows_tile(const Rep &R, const Tile &t, int *rows, int *coun
                 for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
                     if (j == k) continue;
                     double dx, dy, dz;
-                    double d2 = sqdist<DIMS>(R, p, ld4(gp(R.pos) + (j)), dx, dy, dz);
+                    double d2 = sqdist<DIMS>(R, p, ld4(gp(R.pos_alt) + (j)), dx, dy, dz);
                     if (d2 <= R.Rv2) {
                         if (cnt < cap) rows[(size_t)gp(R.orig)[k] * cap + cnt] = gp(R.orig)[j];
                         cnt++;
@@ -2658,15 +2677,6 @@ int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipSt
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_kick_drift, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, ahead ? 1 : 0);
     return check(hipGetLastError());
-}
-
-int launch_drift_finish(const Rep *reps, int nreps, int sysmask, hipStream_t s)
-{
-    (void)reps;
-    (void)nreps;
-    (void)sysmask;
-    (void)s;
-    return 0;  // the drift publishes its maximum with atomics; nothing to finish
 }
 
 int launch_drift_brownian(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, bool ahead)

@@ -166,6 +166,9 @@ class OutputStruct:
         self.q4 = 0.0
         self.q6 = 0.0
         self.F2s = 0.0
+        # the last log step's calculate_full_energy_no_neigh (the reference
+        # prints it as E_pot_test without storing it, src/MD.jl:623, 628)
+        self.E_pot_test = None
 
 
 class NeighborListStruct:
@@ -618,14 +621,21 @@ def read_restart(file, timestep, *, precision="fp64", cell_divisor=1):
     return arrays, parameters, output, neighborlist
 
 
+ENERGY_TEST_MAX_N = 200000  # default size limit of the O(N^2) energy check at log steps
+
+
 def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, restarted=False,
                                equilibrate_velocities=True, new_file=True, noise="device", writer=None,
-                               log=print):
+                               log=print, energy_test=None):
     """src/MD.jl:680-719.  The step loop runs on the device in chunks between
     host-visible events: save steps (which rebuild the lists), logdata steps
     and -- if ``parameters.callback`` is not None -- every step (the callback
     then sees the downloaded state).  ``noise``: "device" (Philox on the GPU)
-    or "host" (numpy normals injected, as randn! would)."""
+    or "host" (numpy normals injected, as randn! would).  ``energy_test``:
+    whether log steps also compute E_pot_test, the O(N^2) all-pairs energy
+    of calculate_full_energy_no_neigh (src/MD.jl:623); None (default) does so
+    for N <= ENERGY_TEST_MAX_N, where one evaluation stays well under a
+    second on the device.  The value is kept in output.E_pot_test."""
     arrays.r_old_array[...] = arrays.r_array
     S = parameters.system
     if not restarted and equilibrate_velocities and not isinstance(S, Brownian):
@@ -648,6 +658,8 @@ def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, rest
     if parameters.callback is not None:
         parameters.callback(arrays, parameters, output, neighborlist)
     start = time.time()
+    if energy_test is None:
+        energy_test = parameters.N <= ENERGY_TEST_MAX_N
     di = parameters.dump_info
     save_steps = sorted(s for s in (di.when_to_save_array if (di and di.save) else []) if s > output.steps_done)
     _upload(arrays, neighborlist, ctx)
@@ -685,12 +697,14 @@ def perform_molecular_dynamics(arrays, parameters, output, neighborlist, *, rest
                 output.q6 = ctx.steinhardt(6, parameters.q_cutoff, True)
                 output.q4 = ctx.steinhardt(4, parameters.q_cutoff, True)
             output.F2s = ctx.f2_self(7.2)
+            output.E_pot_test = ctx.energy_no_neigh() if energy_test else None
             if log is not None:
                 kT = output.kinetic_energy * 2 / S.dims / parameters.N
                 qs = f"q4 = {output.q4:.3f}, q6 = {output.q6:.3f}, " if S.dims == 3 else ""
+                et = f"E_pot_test = {output.E_pot_test:.12f}, " if energy_test else ""
                 log(f"{output.steps_done}/{parameters.N_steps}, "
                     f"E = {output.potential_energy + output.kinetic_energy:.2f}, "
-                    f"E_pot = {output.potential_energy:.12f}, E_kin = {output.kinetic_energy:.2f}, "
+                    f"E_pot = {output.potential_energy:.12f}, {et}E_kin = {output.kinetic_energy:.2f}, "
                     f"kT = {kT:.3f}, {qs}F2s = {output.F2s:.3f}, elapsed = {time.time() - start:.2f}s")
             if output.q6 > 0.2:  # src/MD.jl:711-713
                 raise RuntimeError("Simulation has crystallized, terminating...")
@@ -707,23 +721,42 @@ def find_inherent_structure(r_array_to_be_minimized, parameters, arrays, neighbo
     iteration's forces) and potential_energy_array are mutated, skin-triggered
     rebuilds are counted in output.  verbose >= 1 prints the reference's
     closing line (iterations, total potential energy, largest displacement);
-    verbose's per-1000-iteration report (>= 2) is not reproduced."""
+    verbose >= 2 also its report after iterations 1, 1001, 2001, ...
+    (src/InherentStructure.jl:57-69): the descent then runs as device calls
+    of 1 and then 1000 iterations, which take the same iterations as one
+    call (the stop test is made after every iteration either way)."""
     r0 = np.array(r_array_to_be_minimized, dtype=np.float64, copy=True)
     arrays.r_array[...] = r0
     update_neighbor_lists(arrays, parameters, output, neighborlist)
     ctx = neighborlist.ctx
     before = ctx.rebuild_count()
-    it, _, converged = ctx.minimize_until(max_iterations, dt, max_step_size, tolerance)
+    L = parameters.box_size
+
+    def maxchange():
+        d = arrays.r_array - r0
+        d -= np.round(d / L) * L
+        return float(np.sqrt(np.max(np.sum(d * d, axis=1)))) if len(d) else 0.0
+
+    if verbose >= 2:
+        it, converged, chunk = 0, False, 1
+        while it < max_iterations and not converged:
+            k = min(chunk, max_iterations - it)
+            done, mf, converged = ctx.minimize_until(k, dt, max_step_size, tolerance)
+            it += done
+            if done == k and it % 1000 == 1:
+                arrays.r_array[...] = ctx.get_state(r=True, v=False, F=False)["r"]
+                compute_potential_energy(arrays, parameters, output, neighborlist)
+                print("(iteration, maxchange, stepsize, maxforce, sum(arrays.potential_energy_array)) = ",
+                      (it, maxchange(), mf * dt, mf, float(np.sum(arrays.potential_energy_array))), sep="")
+            chunk = 1000
+    else:
+        it, _, converged = ctx.minimize_until(max_iterations, dt, max_step_size, tolerance)
     _sync_rebuilds(output, ctx, before)
     _download(arrays, neighborlist, ctx)
     if not converged:
         raise RuntimeError("Not converged")
     compute_potential_energy(arrays, parameters, output, neighborlist)
     if verbose >= 1:
-        L = parameters.box_size
-        d = arrays.r_array - r0
-        d -= np.round(d / L) * L
-        maxchange = float(np.sqrt(np.max(np.sum(d * d, axis=1)))) if len(d) else 0.0
         print("after ", it, " iterations, the potential energy is ", float(np.sum(arrays.potential_energy_array)),
-              ". Maximal total displacement = ", maxchange, sep="")
+              ". Maximal total displacement = ", maxchange(), sep="")
     return arrays.r_array.copy()

@@ -4,17 +4,22 @@ Jobs (a replica specification plus a step count) arrive over time.  The
 server keeps one running replica batch (simulationcode_amd._native.Batch) and
 loops:
   1. jobs whose arrival time has passed join the wait queue;
-  2. waiting jobs are admitted in arrival order while a replica slot and the
-     HBM budget (sc_replica_device_bytes) allow; admitted lattice starts are
-     relaxed together in a staging batch (capped steepest descent), then enter
-     the running batch with their own velocities;
-  3. the running batch advances by a chunk = min(steps left over running
-     jobs, max_chunk), so newly arrived work is admitted between chunks while
-     other jobs keep their device-resident state;
-  4. finished jobs are downloaded and retired, freeing slot and budget.
-A job's trajectory depends only on its own specification (per-replica
-Philox noise, per-replica reductions, chunk boundaries that do not change the
-arithmetic), so its result is bitwise the same alone and in the mix.
+  2. the running batch's next chunk = min(steps left over running jobs,
+     max_chunk) is enqueued on its stream and runs on the device;
+  3. meanwhile waiting jobs are admitted in arrival order while a replica
+     slot and the HBM budget (sc_replica_device_bytes) allow, and their
+     lattice starts are relaxed together in a staging batch on its own
+     stream (capped steepest descent), so admission work overlaps the chunk
+     instead of stalling it;
+  4. after the chunk, finished jobs are downloaded and retired (freeing slot
+     and budget), and the relaxed arrivals enter the running batch with their
+     own velocities, their state staying on the device.
+A job whose replica fails (neighbour-list overflow, non-finite state) at
+admission or while running is retired with its error code; the others are
+independent and continue.  A job's trajectory depends only on its own
+specification (per-replica Philox noise, per-replica reductions, chunk
+boundaries that do not change the arithmetic), so its result is bitwise the
+same alone and in the mix.
 """
 import collections
 import time
@@ -68,7 +73,17 @@ class Server:
             job._bytes = nat.replica_device_bytes(p, D)
         return job._bytes
 
-    def _admit(self):
+    def _fail(self, job, code, release=True):
+        if release:
+            self.used_bytes -= job._bytes
+        job.error = code
+        job.left = 0
+        job.done = self.clock()
+        self.finished.append(job)
+
+    def _stage(self):
+        """Admission decisions and the relaxation of the admitted lattice
+        starts in a staging batch; returns [(job, relaxed r)] ready to join."""
         group = []
         while self.waiting and len(self.running) + len(group) < self.max_replicas:
             job = self.waiting[0]
@@ -79,24 +94,52 @@ class Server:
             self.used_bytes += need
             group.append(job)
         if not group:
-            return
+            return []
         staging = nat.Batch(len(group) + 1)
         sid = []
         for job in group:
             p, D, r, v = job._prep
-            sid.append(staging.add(p, D, r, None))
-        staging.minimize(*RELAX)
+            try:
+                sid.append((job, staging.add(p, D, r, None)))
+            except nat.ScError as e:
+                if e.code not in (nat.SC_ERR_NONFINITE, nat.SC_ERR_OVERFLOW):
+                    raise
+                self._fail(job, e.code)
+        try:
+            staging.minimize(*RELAX)
+        except nat.ScError as e:  # one replica failed its relaxation: the others are unaffected
+            if e.code not in (nat.SC_ERR_NONFINITE, nat.SC_ERR_OVERFLOW):
+                raise
+        ready = []
+        for job, s in sid:
+            code = staging.replica_error(s)
+            if code != nat.SC_OK:
+                self._fail(job, code)
+                continue
+            ready.append((job, staging.get_state(s)["r"]))
+        staging.close()
+        return ready
+
+    def _join(self, ready):
         now = self.clock()
-        for job, s in zip(group, sid):
+        for job, r_relaxed in ready:
             p, D, r, v = job._prep
-            r_relaxed = staging.get_state(s)["r"]
-            job.rid = self.batch.add(p, D, r_relaxed, v)
+            del job._prep
+            try:
+                job.rid = self.batch.add(p, D, r_relaxed, v)
+            except nat.ScError as e:
+                if e.code not in (nat.SC_ERR_NONFINITE, nat.SC_ERR_OVERFLOW):
+                    raise
+                self._fail(job, e.code)
+                continue
             job.admitted = now
             self.running[job.rid] = job
-            del job._prep
-        staging.close()
         self.peak_bytes = max(self.peak_bytes, self.used_bytes)
         self.max_running = max(self.max_running, len(self.running))
+
+    def _admit(self):
+        """Stage and join at once (used when nothing is running)."""
+        self._join(self._stage())
 
     def _fail_broken(self):
         """Retire replicas whose state went non-finite or overflowed their
@@ -106,11 +149,7 @@ class Server:
             if code != nat.SC_OK:
                 job = self.running.pop(rid)
                 self.batch.remove(rid)
-                self.used_bytes -= job._bytes
-                job.error = code
-                job.left = 0
-                job.done = self.clock()
-                self.finished.append(job)
+                self._fail(job, code)
 
     def _retire(self):
         now = self.clock()
@@ -133,14 +172,17 @@ class Server:
             now = self.clock()
             while pending and pending[0].arrival <= now:
                 self.waiting.append(pending.popleft())
-            self._admit()
             if not self.running:
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - self.clock(), 0.01)))
-                continue
+                self._admit()
+                if not self.running:
+                    if pending and not self.waiting:
+                        time.sleep(max(0.0, min(pending[0].arrival - self.clock(), 0.01)))
+                    continue
             k = min(min(j.left for j in self.running.values()), self.max_chunk)
+            self.batch.step_async(k)
+            ready = self._stage()  # overlaps the chunk on the device
             try:
-                self.batch.step(k)
+                self.batch.wait()
             except nat.ScError as e:
                 if e.code not in (nat.SC_ERR_NONFINITE, nat.SC_ERR_OVERFLOW):
                     raise
@@ -149,6 +191,7 @@ class Server:
                 j.left -= k
                 self.particle_steps += j.particles * k
             self._retire()
+            self._join(ready)
         return self.finished
 
     def close(self):
@@ -175,6 +218,36 @@ def make_jobs(n, rate, seed, **kw):
     return jobs
 
 
+def _latency_record(srv, done, lam, makespan):
+    failed = sum(1 for j in done if j.error is not None)
+    lat = np.array([j.done - j.arrival for j in done if j.error is None])
+    return {"rate_jobs_per_s": lam, "jobs": len(done), "failed_jobs": failed,
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "particle_steps_per_s": srv.particle_steps / makespan,
+            "makespan_s": makespan, "particle_steps": srv.particle_steps, "max_running": srv.max_running}
+
+
+def summary(args, log, jobs=32, rate=8.0, steps_hi=10000):
+    """A bounded served run for the default benchmark line: `jobs` Poisson
+    arrivals at `rate` per second of the served workload (N log-uniform in
+    [10^3, 2 10^5], dims 2/3, the three dynamics), steps log-uniform in
+    [10^3, steps_hi] (the stated range reaches 10^5; capped so that the run
+    stays within seconds), launch choices as for the batched workload (cell
+    divisor 1, build 256 x 16).  p50/p99 are arrival-to-result latencies."""
+    stream = make_jobs(jobs, rate, seed=args.seed + 8000, steps_hi=steps_hi)
+    srv = Server(max_chunk=1000, cell_divisor=1, build_shape=(256, 16))
+    t0 = time.perf_counter()
+    done = srv.run(stream, t0=t0)
+    makespan = time.perf_counter() - t0
+    srv.close()
+    rec = _latency_record(srv, done, rate, makespan)
+    rec.update({"steps_range": [1000, steps_hi], "N_range": [1000, 200000], "dtype": "mixed",
+                "cell_divisor": 1, "build_shape": [256, 16]})
+    log(f"[bench] served: {jobs} jobs at {rate}/s, p50 {rec['p50_latency_s']:.2f}s p99 {rec['p99_latency_s']:.2f}s, "
+        f"{rec['particle_steps_per_s']:.3g} particle-steps/s")
+    return rec
+
+
 def bench(args, log):
     """Sweep arrival rates; per rate report p50/p99 completion latency
     (arrival to result) and aggregate particle-steps/s over the makespan."""
@@ -193,12 +266,7 @@ def bench(args, log):
         t0 = time.perf_counter()
         done = srv.run(jobs, t0=t0)
         makespan = time.perf_counter() - t0
-        failed = sum(1 for j in done if j.error is not None)
-        lat = np.array([j.done - j.arrival for j in done if j.error is None])
-        rec = {"rate_jobs_per_s": lam, "jobs": len(done), "failed_jobs": failed, "p50_latency_s": float(np.percentile(lat, 50)),
-               "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
-               "particle_steps_per_s": srv.particle_steps / makespan, "makespan_s": makespan,
-               "particle_steps": srv.particle_steps}
+        rec = _latency_record(srv, done, lam, makespan)
         log(f"[serve] rate {lam}: p50 {rec['p50_latency_s']:.2f}s p99 {rec['p99_latency_s']:.2f}s "
             f"{rec['particle_steps_per_s']:.3g} p-s/s")
         sweep.append(rec)

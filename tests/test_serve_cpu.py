@@ -32,6 +32,15 @@ class FakeBatch:
     def step(self, k):
         self.events.append(("step", k, tuple(sorted(self.live))))
 
+    def step_async(self, k):
+        self.step(k)
+
+    def wait(self):
+        pass
+
+    def replica_error(self, rid):
+        return 0
+
     def remove(self, rid):
         self.events.append(("remove", rid))
         del self.live[rid]
@@ -88,3 +97,42 @@ def test_latency_is_arrival_to_completion(monkeypatch):
     done = srv.run(_jobs(3), t0=0.0)
     for j in done:
         assert j.done >= j.admitted >= j.arrival
+
+
+class FailingBatch(FakeBatch):
+    """The replica of the second job added to the running batch fails during
+    its first chunk."""
+
+    def __init__(self, *a, **kw):
+        super().__init__(*a, **kw)
+        self.bad = None
+        self.stepped = False
+
+    def add(self, p, D, r, v=None):
+        rid = super().add(p, D, r, v)
+        if self.n == 2:
+            self.bad = rid
+        return rid
+
+    def step_async(self, k):
+        super().step_async(k)
+        self.stepped = True
+
+    def wait(self):
+        if self.stepped and self.bad in self.live:
+            raise serve.nat.ScError(serve.nat.SC_ERR_NONFINITE, "replica 1: non-finite position")
+
+    def replica_error(self, rid):  # a staging batch (never stepped) reports no error
+        return serve.nat.SC_ERR_NONFINITE if self.stepped and rid == self.bad else 0
+
+
+def test_failed_job_is_retired_and_the_others_complete(monkeypatch):
+    monkeypatch.setattr(serve.nat, "Batch", FailingBatch)
+    monkeypatch.setattr(serve.nat, "replica_device_bytes", lambda p, D: 1)
+    srv = serve.Server(max_chunk=100)
+    done = srv.run(_jobs(3))
+    by_id = {j.id: j for j in done}
+    assert sorted(by_id) == [0, 1, 2]
+    assert by_id[1].error == serve.nat.SC_ERR_NONFINITE
+    assert by_id[0].error is None and by_id[2].error is None
+    assert srv.used_bytes == 0 and not srv.running
