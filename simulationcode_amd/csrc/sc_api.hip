@@ -360,11 +360,21 @@ int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, c
     if (v && (rc = upload_field(c, v, FLD_V))) return rc;
     if (F && (rc = upload_field(c, F, FLD_F))) return rc;
     if (D) {
-        if ((rc = upload_field(c, D, FLD_D))) return rc;
         // the windowed build bounds every pair's half-list radius by the
-        // largest diameter (swap moves keep the set of diameters)
+        // largest diameter (swap moves keep the set of diameters), and the
+        // mixed form's fixed-point diameters need a scale that holds it
+        // (rescaled before the upload, which rewrites every diameter)
         float dm = c->h.dmax;
-        for (int64_t i = 0; i < c->h.n; i++) dm = std::max(dm, (float)D[i]);
+        double dmx = 0.0;
+        for (int64_t i = 0; i < c->h.n; i++) {
+            dm = std::max(dm, (float)D[i]);
+            dmx = std::max(dmx, D[i]);
+        }
+        if (c->h.spos && dmx * c->h.dqinv >= 4294967295.0) {
+            sc::set_diameter_scale(c->h, dmx);
+            if ((rc = push_rep(c))) return rc;
+        }
+        if ((rc = upload_field(c, D, FLD_D))) return rc;
         if (dm > c->h.dmax) {
             c->h.dmax = dm;
             // beyond zeta * max(D) = 2 the windowed build's FP32 margin no

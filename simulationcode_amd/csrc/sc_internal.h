@@ -8,6 +8,8 @@
 // HBM layout per replica (n particles, slots sorted by cell at each rebuild):
 //   pos    double4[n]   {x, y, z, D}   (z = 0 in 2D); D rides with r because
 //                                      every pair evaluation needs both
+//   spos   uint4[n]     mixed precision: the same in 32-bit fixed point, the
+//                                      16-byte record its pair loop gathers
 //   v, F, disp          SoA doubles, component c at [c*n + i]
 //   orig   int[n]       original (host) index of the particle in slot i
 //   nn     int[n]       short-list length of slot i
@@ -93,6 +95,11 @@ struct Rep {
     double c0, c2, c4, zeta;
     double kBT, gamma, dt, m;
     double min_tol;      // minimiser: stop once max |F| <= min_tol (< 0: never)
+    // mixed precision: each slot's position and diameter as 32-bit fixed point
+    // (spos), u = x * uq mod 2^32 (uq = 2^32 / L, so a pair's minimum image is
+    // the wrapping difference u_i - u_j read as a signed integer, times lq =
+    // L / 2^32) and w = D * dqinv (dqinv a power of two, D = w * dq)
+    double uq, lq, dq, dqinv;
     uint64_t seed;
     // ---- injected noise (NULL => Philox): block s at noise + (step - noise_step0) * n * dims,
     //      for 0 <= step - noise_step0 < noise_blocks
@@ -101,6 +108,7 @@ struct Rep {
     long long noise_blocks;
     // ---- device buffers
     double4 *pos, *pos_alt;
+    uint4 *spos;         // mixed precision only (else NULL): {ux, uy, uz, w} per slot, kept with pos
     double *v, *v_alt, *F, *disp;
     int *orig, *orig_alt;
     int *cell_of, *rank, *cell_cnt, *cell_start, *chunk_off, *key;
@@ -187,6 +195,10 @@ namespace sc {
 bool params_valid(const sc_params *p);
 int64_t replica_bytes(const sc_params *p, const double *D);
 int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *> &allocs, std::string &err);
+// fixed-point diameter scale of the mixed form for diameters up to maxD, and
+// one diameter in it (the device's rint(D * dqinv), bit for bit)
+void set_diameter_scale(Rep &h, double maxD);
+unsigned diameter_fixed(double D, double dqinv);
 void free_all(std::vector<void *> &allocs);
 }  // namespace sc
 

@@ -33,6 +33,7 @@ __device__ __forceinline__ SC_AS1 T *gp(T *p)
 // double4 loads / stores through global pointers (the vector type's own
 // copy operations take generic references)
 typedef double sc_dbl4v __attribute__((ext_vector_type(4)));
+typedef unsigned sc_u32x4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ double4 ld4(const SC_AS1 double4 *p)
 {
     const sc_dbl4v v = *(const SC_AS1 sc_dbl4v *)p;
@@ -280,45 +281,63 @@ __device__ inline double pair_fdivr64(const Rep &R, double rij2, double Di, doub
     }
 }
 
-// FP32 force/r (mixed precision): same formulas, FP32 arithmetic; the
-// hardware reciprocal is refined by one Newton step (error ~1 ulp).
-__device__ inline float rcp32(float x)
+// Reciprocal for the mixed form: the FP32 hardware estimate (relative error
+// ~2^-23) refined by one FP64 Newton step r <- r + r (1 - x r), which squares
+// the error (~1e-14), for the normal, finite x met here.
+__device__ inline double rcp_mixed(double x)
 {
-    float r = __builtin_amdgcn_rcpf(x);
-    float e = fmaf(-x, r, 1.0f);
-    return fmaf(r, e, r);
+    const double r = (double)__builtin_amdgcn_rcpf((float)x);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
+// Mixed form, one pair from the fixed-point records qi, qj: separation from
+// the wrapping differences of the 32-bit coordinates (the minimum image,
+// exact), products and sums in FP64 (no slower than FP32 on this hardware
+// without packed FP32), the reciprocals from FP32 estimates.  The force law
+// and cutoff tests are those of pair_fdivr64 (src/MD.jl:120-126, 175-180,
+// 228-234, 299-307).  Accumulates force/r times the separation into f.
 template <int DIMS, int POT>
-__device__ inline float pair_fdivr32(const Rep &R, float rij2, float Di, float Dj)
+__device__ inline void pair_mixed(const Rep &R, const uint4 qi, const double Di, const uint4 qj, double (&f)[3])
 {
+    const double dx = (double)(int)(qi.x - qj.x) * R.lq;
+    const double dy = (double)(int)(qi.y - qj.y) * R.lq;
+    const double dz = DIMS == 3 ? (double)(int)(qi.z - qj.z) * R.lq : 0.0;
+    double r2 = fma(dy, dy, dx * dx);
+    if (DIMS == 3) r2 = fma(dz, dz, r2);
+    double fs;
     if (POT == 2) {
-        float md = (Di + Dj) * 0.5f * (1.0f - (float)R.zeta * fabsf(Di - Dj));
-        float md2 = md * md;
-        float inv_md2 = rcp32(md2);
-        float invxi2 = rij2 * inv_md2;
-        float invxi4 = invxi2 * invxi2;
-        float xi14 = rcp32(invxi4 * invxi4 * invxi4 * invxi2);
-        float br = fmaf(2.0f * (float)R.c4, invxi2, (float)R.c2);
-        br = fmaf(-6.0f, xi14, br);
-        return ((float)R.fc2 * md2 < rij2) ? 0.0f : -2.0f * inv_md2 * br;
+        const double Dj = (double)qj.w * R.dq;
+        const double md = (Di + Dj) * fma(-0.5 * R.zeta, fabs(Di - Dj), 0.5);
+        const double md2 = md * md;
+        const double inv_md2 = rcp_mixed(md2);
+        const double x = r2 * inv_md2;  // (r / sigma_ij)^2
+        const double x2 = x * x, x4 = x2 * x2;
+        const double xi14 = rcp_mixed(x4 * x2 * x);
+        fs = (R.fc2 * md2 < r2) ? 0.0 : -2.0 * inv_md2 * fma(-6.0, xi14, fma(2.0 * R.c4, x, R.c2));
     } else if (DIMS == 3) {
-        float inv_r2 = rcp32(rij2);
-        float xi2 = (float)R.lj_s2 * inv_r2;
-        float xi6 = xi2 * xi2 * xi2;
-        float xi12 = xi6 * xi6;
-        return ((float)R.fc2 < rij2) ? 0.0f : (float)R.eps * fmaf(48.0f, xi12, -24.0f * xi6) * inv_r2;
+        const double inv_r2 = rcp_mixed(r2);
+        const double xi2 = R.lj_s2 * inv_r2;
+        const double xi6 = xi2 * xi2 * xi2;
+        fs = (R.fc2 < r2) ? 0.0 : R.eps * fma(48.0, xi6 * xi6, -24.0 * xi6) * inv_r2;
     } else {
-        float md2 = (float)(R.lj_md * R.lj_md);
-        if ((float)R.fc2 * md2 > rij2) {
-            float inv_r2 = rcp32(rij2);
-            float xi2 = md2 * inv_r2;
-            float xi6 = xi2 * xi2 * xi2;
-            float xi12 = xi6 * xi6;
-            return (float)R.eps * fmaf(48.0f, xi12, -24.0f * xi6) * inv_r2;
+        const double md2 = R.lj_md * R.lj_md;
+        fs = 0.0;
+        if (R.fc2 * md2 > r2) {
+            const double inv_r2 = rcp_mixed(r2);
+            const double xi2 = md2 * inv_r2;
+            const double xi6 = xi2 * xi2 * xi2;
+            fs = R.eps * fma(48.0, xi6 * xi6, -24.0 * xi6) * inv_r2;
         }
-        return 0.0f;
     }
+    f[0] = fma(fs, dx, f[0]);
+    f[1] = fma(fs, dy, f[1]);
+    if (DIMS == 3) f[2] = fma(fs, dz, f[2]);
+}
+
+__device__ __forceinline__ uint4 ldq(const uint4 *p)
+{
+    const sc_u32x4v v = *(const SC_AS1 sc_u32x4v *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
 }
 
 // Pair energy with the cutoff test of find_energy's inner loops
@@ -414,6 +433,20 @@ __device__ inline void cell_of_pos(const Rep &R, double4 p, int (&cc)[3])
 
 __device__ inline int wrapc(int c, int nc) { return c < 0 ? c + nc : (c >= nc ? c - nc : c); }
 
+// Mixed precision: the 32-bit fixed-point record of slot k, written wherever
+// pos is.  A coordinate maps to u = rint(x * 2^32 / L) mod 2^32 (modular, so a
+// position a rounding outside [0, L) still lands on its periodic image); the
+// diameter to w = rint(D * dqinv), exact to the scale's resolution.
+__device__ inline unsigned fixed_coord(double x, double uq) { return (unsigned)(long long)rint(x * uq); }
+
+__device__ inline void put_shadow(const Rep &R, int k, const double4 p)
+{
+    if (!R.spos) return;
+    *(SC_AS1 sc_u32x4v *)(R.spos + k) =
+        sc_u32x4v{fixed_coord(p.x, R.uq), fixed_coord(p.y, R.uq), R.dims == 3 ? fixed_coord(p.z, R.uq) : 0u,
+                  (unsigned)(long long)rint(p.w * R.dqinv)};
+}
+
 // ====================================================================
 // Integrator kernels
 // ====================================================================
@@ -450,7 +483,9 @@ __device__ void kick_drift_tile(const Rep &R, const Tile &t)
             gp(R.v)[c * n + i] = vc;
             gp(R.disp)[c * n + i] = dc;
         }
-        st4(gp(R.pos) + (i), make_double4(pc[0], pc[1], pc[2], p.w));
+        const double4 pn = make_double4(pc[0], pc[1], pc[2], p.w);
+        st4(gp(R.pos) + (i), pn);
+        put_shadow(R, i, pn);
         s2 = s;
         if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }
@@ -498,7 +533,9 @@ __device__ void drift_brownian_tile(const Rep &R, const Tile &t)
             gp(R.F)[c * n + i] = fc;
             gp(R.disp)[c * n + i] = dc;
         }
-        st4(gp(R.pos) + (i), make_double4(pc[0], pc[1], pc[2], p.w));
+        const double4 pn = make_double4(pc[0], pc[1], pc[2], p.w);
+        st4(gp(R.pos) + (i), pn);
+        put_shadow(R, i, pn);
         s2 = s;
         if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }
@@ -568,45 +605,21 @@ __device__ void force_tile(const Rep &R, const Tile &t)
                 if (DIMS == 3) f[2] += fs * dz;
             }
         } else {
-            // FP64 minimum image; FP32 r^2 and force law; the FP32 force/r
-            // multiplies the FP64 separation and accumulates in FP64.
-            const float Di = (float)pi.w;
+            // mixed: 16-byte fixed-point records (one load per pair)
+            const uint4 qi = ldq(R.spos + i);
+            const double Di = (double)qi.w * R.dq;
             int k = 0;
             for (; k + 4 <= cnt; k += 4) {
                 int jj[4];
-                double4 pp[4];
+                uint4 qq[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) jj[u] = nb[(size_t)(k + u) * n];
 #pragma unroll
-                for (int u = 0; u < 4; u++) pp[u] = ld4(gp(R.pos) + (jj[u]));
+                for (int u = 0; u < 4; u++) qq[u] = ldq(R.spos + jj[u]);
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const double dx = min_image(pi.x - pp[u].x, R.L, R.invL);
-                    const double dy = min_image(pi.y - pp[u].y, R.L, R.invL);
-                    const double dz = DIMS == 3 ? min_image(pi.z - pp[u].z, R.L, R.invL) : 0.0;
-                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-                    float rij2 = fmaf(fy, fy, fx * fx);
-                    if (DIMS == 3) rij2 = fmaf(fz, fz, rij2);
-                    const double fs = (double)pair_fdivr32<DIMS, POT>(R, rij2, Di, (float)pp[u].w);
-                    f[0] += fs * dx;
-                    f[1] += fs * dy;
-                    if (DIMS == 3) f[2] += fs * dz;
-                }
+                for (int u = 0; u < 4; u++) pair_mixed<DIMS, POT>(R, qi, Di, qq[u], f);
             }
-            for (; k < cnt; k++) {
-                const int j = nb[(size_t)k * n];
-                const double4 pj = ld4(gp(R.pos) + (j));
-                const double dx = min_image(pi.x - pj.x, R.L, R.invL);
-                const double dy = min_image(pi.y - pj.y, R.L, R.invL);
-                const double dz = DIMS == 3 ? min_image(pi.z - pj.z, R.L, R.invL) : 0.0;
-                const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-                float rij2 = fmaf(fy, fy, fx * fx);
-                if (DIMS == 3) rij2 = fmaf(fz, fz, rij2);
-                const double fs = (double)pair_fdivr32<DIMS, POT>(R, rij2, Di, (float)pj.w);
-                f[0] += fs * dx;
-                f[1] += fs * dy;
-                if (DIMS == 3) f[2] += fs * dz;
-            }
+            for (; k < cnt; k++) pair_mixed<DIMS, POT>(R, qi, Di, ldq(R.spos + nb[(size_t)k * n]), f);
         }
         // add_new_random_force! (src/MD.jl:30-43 Langevin, :67-78 Brownian)
         if (R.sys != 0) {
@@ -1126,6 +1139,7 @@ __device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, i
     if (nfull > R.cap) gatomic_or(&gp(R.dyn)->err, SC_EBIT_OVERFLOW);
     gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
     st4(gp(R.pos) + (k), p);
+    put_shadow(R, k, p);
     const bool keepF = (gp(R.dyn)->rebuild & REBUILD_KEEP_F) != 0;
     for (int c = 0; c < DIMS; c++) {
         gp(R.v)[c * n + k] = gp(R.v_alt)[c * n + k];
@@ -2352,11 +2366,13 @@ __global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Re
         p.y = st[(size_t)o * d + 1];
         p.z = d == 3 ? st[(size_t)o * d + 2] : 0.0;
         st4(gp(R.pos) + (k), p);
+        put_shadow(R, k, p);
     } break;
     case FLD_D: {
         double4 p = ld4(gp(R.pos) + (k));
         p.w = st[o];
         st4(gp(R.pos) + (k), p);
+        put_shadow(R, k, p);
     } break;
     case FLD_V:
     case FLD_F:
@@ -2422,7 +2438,9 @@ __device__ void min_move_tile(const Rep &R, const Tile &t, double dt)
             fin = fin && isfinite(rn);
             gp(R.disp)[c * n + i] = dc;
         }
-        st4(gp(R.pos) + (i), make_double4(pc[0], pc[1], pc[2], p.w));
+        const double4 pn = make_double4(pc[0], pc[1], pc[2], p.w);
+        st4(gp(R.pos) + (i), pn);
+        put_shadow(R, i, pn);
         s2 = s;
         if (!fin) gatomic_or(&dyn->err, SC_EBIT_NONFINITE);
     }

@@ -55,9 +55,24 @@ int64_t replica_bytes(const sc_params *p, const double *D)
     b += 5 * n * 4;                   // orig, orig_alt, cell_of, rank, key
     b += (2 * ncell + 1) * 4 + 4 * (1 + SC_CHUNK_SPLIT) * ((ncell + SC_CHUNK - 1) / SC_CHUNK);
     b += n * 4 + n * p->maxneighbours * 4;  // nn, nbr
+    if (p->precision == SC_PREC_MIXED) b += n * 16;  // spos
     b += ntiles * 8 * 8 + n * 3 * 8 + 4096;  // partials, staging, dyn
     return b;
 }
+
+// Fixed-point diameters of the mixed form: w = D * 2^(31-e) with D_max <
+// 2^e, so w < 2^31 and diameters up to twice the largest still fit 32 bits;
+// D = w * 2^(e-31) then resolves D to 2^-30 of its power-of-two bound
+// (about 1e-9 relative at D ~ 1).
+void set_diameter_scale(Rep &h, double maxD)
+{
+    int e = 0;
+    (void)std::frexp(maxD > 0.0 ? maxD : 1.0, &e);  // maxD = f 2^e, f in [0.5, 1)
+    h.dqinv = std::ldexp(1.0, 31 - e);
+    h.dq = std::ldexp(1.0, e - 31);
+}
+
+unsigned diameter_fixed(double D, double dqinv) { return (unsigned)(long long)std::rint(D * dqinv); }
 
 static int alloc_into(std::vector<void *> &allocs, void **ptr, size_t bytes, std::string &err)
 {
@@ -170,6 +185,9 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     h.m = p->m;
     h.min_tol = -1.0;
     h.seed = p->seed;
+    h.uq = std::ldexp(1.0, 32) / p->box_size;
+    h.lq = p->box_size * std::ldexp(1.0, -32);
+    set_diameter_scale(h, maxD);
     long long nc;
     int srange;
     cell_grid(p, Rv, nc, srange);
@@ -231,6 +249,8 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     rc |= A(allocs, &h.part, (size_t)h.ntiles * 8, err);
     rc |= A(allocs, &h.stage, (size_t)n * 3, err);
     rc |= A(allocs, &h.dyn, 1, err);
+    h.spos = nullptr;
+    if (h.prec == SC_PREC_MIXED) rc |= A(allocs, &h.spos, n, err);
     if (rc != SC_OK) return SC_ERR_OOM;
 
     std::vector<int> ident(n);
@@ -252,6 +272,11 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     ok &= hipMemset(h.chunk_off, 0, sizeof(int) * nchunks) == hipSuccess;
     ok &= hipMemset(h.chunk_cnt, 0, sizeof(int) * nchunks * SC_CHUNK_SPLIT) == hipSuccess;
     ok &= hipMemcpy(h.dyn, &dyn, sizeof(RepDyn), hipMemcpyHostToDevice) == hipSuccess;
+    if (h.spos) {  // the shadow of the initial state (positions 0, diameters D)
+        std::vector<uint4> sp(n);
+        for (int i = 0; i < n; i++) sp[i] = make_uint4(0u, 0u, 0u, diameter_fixed(D[i], h.dqinv));
+        ok &= hipMemcpy(h.spos, sp.data(), sizeof(uint4) * n, hipMemcpyHostToDevice) == hipSuccess;
+    }
     if (!ok) {
         err = "replica initialisation copy failed";
         return SC_ERR_HIP;

@@ -387,6 +387,57 @@ def test_mixed_precision_force_error(kind, dims):
     assert err <= 1e-5, err
 
 
+def test_mixed_precision_force_error_large_thermalised():
+    """The large-system shape (N = 10^6 polydisperse 3D, relaxed, then 300
+    Newtonian steps from kBT = 0.2): the mixed form's forces within 1e-5 of
+    the mean force magnitude of the oracle's FP64 forces, particle by
+    particle."""
+    N, dims = 10 ** 6, 3
+    op, r, D, v, rc = H.make_system("berthier", N, dims, seed=20261207, relax=False, maxneighbours=100, kBT=0.2)
+    ctx = make_ctx(op, rc, r, v, D, cell_divisor=2)
+    ctx.update_neighbor_lists()
+    ctx.minimize(300, 0.005, 0.02)
+    ctx.set_state(v=v, F=np.zeros_like(v))
+    ctx.update_neighbor_lists()
+    ctx.calculate_new_forces(None)
+    ctx.do_time_steps(300)
+    g = ctx.get_state()
+    ctx.close()
+    cm = make_ctx(op, rc, g["r"], g["v"], D, precision=nat.PREC_MIXED, cell_divisor=2)
+    cm.update_neighbor_lists()
+    cm.calculate_new_forces(None)
+    Fm = cm.get_state(r=False, v=False, F=True)["F"]
+    lst = oracle.build_lists(op, g["r"], D, cells=True)
+    Fo = oracle.calculate_new_forces(op, g["r"], D, lst)
+    fn = np.linalg.norm(Fo, axis=1)
+    err = float(np.max(np.linalg.norm(Fm - Fo, axis=1)) / fn.mean())
+    assert err <= 1e-5, err
+
+
+def test_mixed_diameters_rescaled_on_upload():
+    """Diameters uploaded above twice the largest at creation exceed the
+    mixed form's fixed-point diameter range: the scale is widened before the
+    upload and forces still follow the oracle (with R_v fixed at creation,
+    as initialize_neighbor_struct fixes it)."""
+    N, dims = 3000, 3
+    op, r, D0, v, rc = H.make_system("berthier", N, dims, seed=71)
+    D1 = D0 * 2.6  # max 4.2 > 4 = twice the power of two above max(D0)
+    assert D1.max() > 4.0 >= 2 * D0.max()
+    out = {}
+    for prec in (nat.PREC_FP64, nat.PREC_MIXED):
+        ctx = make_ctx(op, rc, r, v, D0, precision=prec)
+        ctx.set_state(D=D1)
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        out[prec] = ctx.get_state(r=False, v=False, F=True)["F"]
+        ctx.close()
+    lst = oracle.build_lists(op, r, D1, cells=False)  # op keeps R_v of D0
+    Fo = oracle.calculate_new_forces(op, r, D1, lst)
+    assert H.force_rel_err(out[nat.PREC_FP64], Fo) <= 1e-12
+    mean_f = np.mean(np.linalg.norm(Fo, axis=1))
+    assert np.max(np.linalg.norm(out[nat.PREC_MIXED] - Fo, axis=1)) / mean_f <= 1e-5
+
+
 def test_large_system_forces_match_oracle():
     """N = 10^6 polydisperse 3D (the large-system shape): relax on the GPU,
     then compare every particle's force with the oracle."""
