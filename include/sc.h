@@ -105,6 +105,18 @@ int sc_update_neighbor_lists(sc_ctx *ctx);
 int sc_get_neighbor_lists(sc_ctx *ctx, int64_t maxneigh, int64_t *nfull, int64_t *full,
                           int64_t *nhalf, int64_t *half);
 
+/* The lists of sc_get_neighbor_lists (same layout and entries), for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving or Monte-Carlo loops -- with no counterpart in the reference.  It
+ * calls no hipMalloc/hipFree: the rows are assembled in a 64 MiB scratch
+ * array of static device memory (each GPU the library runs on holds it until
+ * the process ends) and copied out.  Entries past each row's count are set
+ * to 0.  Each requested list needs N * (maxneigh + 1) 8-byte words of the
+ * scratch; beyond 8388576 words in total the call returns SC_ERR_ARG.  Calls
+ * on contexts of one GPU share the scratch: calls that overlap race. */
+int sc_get_neighbor_lists_noalloc(sc_ctx *ctx, int64_t maxneigh, int64_t *nfull, int64_t *full,
+                                  int64_t *nhalf, int64_t *half);
+
 /* calculate_new_forces! -- src/MD.jl:262-316, including
  * add_new_random_force! (:30-82).  noise: N x dims AoS standard normals
  * (what randn! would have produced), or NULL for on-device Philox noise. */

@@ -41,7 +41,7 @@ _ERRNAMES = {
 # Every symbol include/sc.h declares (checked by the CPU tests).
 EXPORTS = (
     "sc_create", "sc_destroy", "sc_last_error", "sc_set_state", "sc_get_state",
-    "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_calculate_new_forces",
+    "sc_update_neighbor_lists", "sc_get_neighbor_lists", "sc_get_neighbor_lists_noalloc", "sc_calculate_new_forces",
     "sc_do_time_steps", "sc_full_energy", "sc_energy_no_neigh", "sc_kinetic_energy",
     "sc_rescale_velocities", "sc_rebuild_count", "sc_minimize", "sc_minimize_until", "sc_philox4x32_10",
     "sc_last_elapsed_ms", "sc_copy_bandwidth", "sc_potential_energies", "sc_set_kernel_timing", "sc_kernel_times", "sc_neighbor_stats",
@@ -141,6 +141,7 @@ def load(path=LIB_PATH):
         "sc_get_state": (ctypes.c_int, [vp, vp, vp, vp, vp, vp]),
         "sc_update_neighbor_lists": (ctypes.c_int, [vp]),
         "sc_get_neighbor_lists": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
+        "sc_get_neighbor_lists_noalloc": (ctypes.c_int, [vp, i64, vp, vp, vp, vp]),
         "sc_calculate_new_forces": (ctypes.c_int, [vp, vp]),
         "sc_do_time_steps": (ctypes.c_int, [vp, i64, vp]),
         "sc_full_energy": (ctypes.c_int, [vp, pd]),

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +19,41 @@
 namespace sc {
 void philox_host(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]);
 }
+
+// Device scratch of sc_get_neighbor_lists_noalloc: the reference-layout rows
+// and counts are assembled here instead of in buffers allocated per call.
+// Every GPU the library runs on holds its own copy for the life of the
+// process; concurrent calls on one GPU share it (and race).
+#define SC_SCRATCH_WORDS (8u << 20)  // 8-byte words: 64 MiB
+#define SC_SCRATCH_FLAG_WORDS 32     // words reserved at the start for flags
+static __device__ __align__(256) unsigned long long scratch[SC_SCRATCH_WORDS];
+
+namespace {
+// scratch's device address on each GPU (the host-side symbol is a stand-in,
+// not an address any kernel may use), looked up once per device
+std::mutex g_scratch_mu;
+unsigned long long *g_scratch_ptr[256] = {};
+
+int scratch_address(int dev, unsigned long long **out, std::string &err)
+{
+    if (dev < 0 || dev >= 256) {
+        err = "device index out of range for the scratch table";
+        return SC_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q) {
+            err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return SC_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned long long *>(q);
+    }
+    *out = g_scratch_ptr[dev];
+    return SC_OK;
+}
+}  // namespace
 
 struct sc_ctx {
     sc_params p{};
@@ -41,6 +77,7 @@ struct sc_ctx {
     double t_force = 0.0, t_drift = 0.0;
     long long n_force = 0, n_drift = 0;
     RepDyn *h_dyn = nullptr;  // pinned copy of the device state, polled between step chunks
+    unsigned long long *scratch = nullptr;  // scratch's address on the context's device
 };
 
 namespace {
@@ -48,6 +85,13 @@ namespace {
 int fail(sc_ctx *c, int code, const std::string &msg)
 {
     if (c) c->err = msg;
+    return code;
+}
+
+// A literal message: assigned into the context's reserved text, no allocation.
+int fail(sc_ctx *c, int code, const char *msg)
+{
+    if (c) c->err.assign(msg);
     return code;
 }
 
@@ -309,7 +353,11 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     sc_ctx *c = new sc_ctx();
     c->p = *p;
     *out = c;
+    c->err.reserve(512);  // error text fits without reallocation (the allocation-free call)
     if (hipStreamCreate(&c->s) != hipSuccess) return fail(c, SC_ERR_HIP, "hipStreamCreate failed");
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (int rc0 = scratch_address(dev, &c->scratch, c->err)) return rc0;
     HIPCHK(c, hipEventCreate(&c->ev0));
     HIPCHK(c, hipEventCreate(&c->ev1));
     Rep &h = c->h;
@@ -464,6 +512,41 @@ int sc_get_neighbor_lists(sc_ctx *c, int64_t maxneigh, int64_t *nfull, int64_t *
         if (nhalf)
             for (int i = 0; i < n; i++) nhalf[i] = cnt[i];
     }
+    return SC_OK;
+}
+
+int sc_get_neighbor_lists_noalloc(sc_ctx *c, int64_t maxneigh, int64_t *nfull, int64_t *full, int64_t *nhalf,
+                                  int64_t *half)
+{
+    if (!c || !c->d_rep || maxneigh < 1 || maxneigh > 1000000) return SC_ERR_ARG;
+    const int64_t n = c->h.n;
+    const bool want_full = nfull || full, want_half = nhalf || half;
+    const uint64_t need = SC_SCRATCH_FLAG_WORDS + (uint64_t)(want_full + want_half) * (uint64_t)n * (maxneigh + 1);
+    if (need > SC_SCRATCH_WORDS)
+        return fail(c, SC_ERR_ARG, "N * (maxneigh + 1) too large for the 64 MiB export scratch "
+                                   "(at most 8388576 words per list requested); use sc_get_neighbor_lists");
+    RepDyn d;
+    int rc = read_dyn(c, &d);
+    if (rc) return rc;
+    if (!d.have_lists) return fail(c, SC_ERR_STATE, "neighbour lists have not been built");
+    int *flag = reinterpret_cast<int *>(c->scratch);
+    long long *base = reinterpret_cast<long long *>(c->scratch + SC_SCRATCH_FLAG_WORDS);
+    int over[2] = {0, 0};
+    for (int h = 0; h < 2; h++) {
+        if (!(h ? want_half : want_full)) continue;
+        long long *rows = base, *cnts = base + (size_t)n * maxneigh;
+        base += (size_t)n * (maxneigh + 1);
+        KCHK(c, sc::launch_ref_rows(c->d_tiles, c->ntiles, c->d_rep, h, rows, cnts, (int)maxneigh, flag + h, c->s));
+        int64_t *dst_rows = h ? half : full, *dst_cnt = h ? nhalf : nfull;
+        if (dst_rows)
+            HIPCHK(c, hipMemcpyAsync(dst_rows, rows, sizeof(int64_t) * (size_t)n * maxneigh, hipMemcpyDeviceToHost, c->s));
+        if (dst_cnt) HIPCHK(c, hipMemcpyAsync(dst_cnt, cnts, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost, c->s));
+    }
+    HIPCHK(c, hipMemcpyAsync(over, flag, sizeof(over), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(c, hipStreamSynchronize(c->s));
+    // each requested list cleared its own flag word first; the other word is stale
+    if (want_full && over[0]) return fail(c, SC_ERR_OVERFLOW, "full list row longer than maxneigh");
+    if (want_half && over[1]) return fail(c, SC_ERR_OVERFLOW, "half list row longer than maxneigh");
     return SC_OK;
 }
 

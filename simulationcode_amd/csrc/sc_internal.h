@@ -179,6 +179,11 @@ int launch_steinhardt(const Tile *tiles, int ntiles, const Rep *reps, int nreps,
 int launch_from_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
 int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int field, hipStream_t s);
 int launch_full_rows(const Tile *tiles, int ntiles, const Rep *reps, int *rows, int *counts, int cap, hipStream_t s);
+// reference-layout Int64 rows (1-based, zero past each count) and counts of
+// the full (half = 0) or half (half = 1) list, written into caller memory;
+// *flag (cleared first) becomes 1 if a row is longer than maxneigh
+int launch_ref_rows(const Tile *tiles, int ntiles, const Rep *reps, int half, long long *rows, long long *counts,
+                    int maxneigh, int *flag, hipStream_t s);
 int launch_min_scale(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, double max_step,
                      hipStream_t s);
 int launch_min_move(const Tile *tiles, int ntiles, const Rep *reps, int nreps, double dt, hipStream_t s);

@@ -1978,6 +1978,74 @@ __global__ __launch_bounds__(SC_TB) void k_full_rows(const Tile *tiles, const Re
         full_rows_tile<2>(R, t, rows, counts, cap);
 }
 
+// Reference-layout rows built on the device (the allocation-free list
+// export): for the slot of particle a, row a of the column-major
+// maxneigh x N Int64 array gets 1-based entries, zeros past its count, and
+// counts[a] its length; a row longer than maxneigh sets flag[0].  full: the
+// rebuild-time full rows of full_rows_tile; half: the pairs of the stored
+// short list with a < b, in list order.
+template <int DIMS>
+__device__ void ref_rows_tile(const Rep &R, const Tile &t, int half, long long *rows, long long *counts,
+                              int maxneigh, int *flag)
+{
+    const int n = R.n, k = t.first + threadIdx.x;
+    if (k >= n) return;
+    const int a = gp(R.orig)[k];
+    SC_AS1 long long *row = gp(rows) + (size_t)a * maxneigh;
+    int cnt = 0;
+    if (half) {
+        const int m = gp(R.nn)[k];
+        for (int q = 0; q < m; q++) {
+            const int b = gp(R.orig)[gp(R.nbr)[(size_t)q * n + k]];
+            if (a < b) {
+                if (cnt < maxneigh) row[cnt] = (long long)b + 1;
+                cnt++;
+            }
+        }
+    } else {
+        const double4 p = ld4(gp(R.pos_alt) + (k));
+        int cc[3];
+        cell_of_pos<DIMS>(R, p, cc);
+        const int s = R.srange;
+        const int zr = DIMS == 3 ? s : 0;
+        for (int oz = -zr; oz <= zr; oz++)
+            for (int oy = -s; oy <= s; oy++)
+                for (int ox = -s; ox <= s; ox++) {
+                    const int c = wrapc(cc[0] + ox, R.nc) +
+                                  R.nc * (wrapc(cc[1] + oy, R.nc) + R.nc * (DIMS == 3 ? wrapc(cc[2] + oz, R.nc) : 0));
+                    for (int j = cstart(R, c), je = cstart(R, c + 1); j < je; j++) {
+                        if (j == k) continue;
+                        double dx, dy, dz;
+                        if (sqdist<DIMS>(R, p, ld4(gp(R.pos_alt) + (j)), dx, dy, dz) <= R.Rv2) {
+                            if (cnt < maxneigh) row[cnt] = (long long)gp(R.orig)[j] + 1;
+                            cnt++;
+                        }
+                    }
+                }
+    }
+    for (int q = cnt; q < maxneigh; q++) row[q] = 0;
+    gp(counts)[a] = cnt;
+    if (cnt > maxneigh) gatomic_or(gp(flag), 1);
+}
+
+__global__ __launch_bounds__(SC_TB) void k_ref_rows(const Tile *tiles, const Rep *reps, int half, long long *rows,
+                                                    long long *counts, int maxneigh, int *flag)
+{
+    const Tile t = tiles[xcd_tile()];
+    const Rep R = reps[t.rep];
+    if (R.dims == 3)
+        ref_rows_tile<3>(R, t, half, rows, counts, maxneigh, flag);
+    else
+        ref_rows_tile<2>(R, t, half, rows, counts, maxneigh, flag);
+}
+
+// Fill kernel (no memset node, so a captured launch sequence replays it).
+__global__ void k_zero_ints(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) gp(p)[i] = 0;
+}
+
 // ====================================================================
 // Energies
 // ====================================================================
@@ -2878,6 +2946,14 @@ int launch_to_host_order(const Tile *tiles, int ntiles, const Rep *reps, int fie
 {
     if (ntiles <= 0) return 0;
     hipLaunchKernelGGL(k_to_host, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, field);
+    return check(hipGetLastError());
+}
+
+int launch_ref_rows(const Tile *tiles, int ntiles, const Rep *reps, int half, long long *rows, long long *counts,
+                    int maxneigh, int *flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, s, flag, 1);
+    hipLaunchKernelGGL(k_ref_rows, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, half, rows, counts, maxneigh, flag);
     return check(hipGetLastError());
 }
 
