@@ -74,12 +74,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1, cell_divisor=0):
+def make_ctx(nat, N, L, D, precision, dt=0.005, kBT=0.2, seed=1, cell_divisor=0, build_mode=0):
     from simulationcode_amd import md
     U = md.Berthier(rc=1.25)
     p = nat.ScParams()
     p.dims, p.potential, p.system, p.precision = 3, nat.POT_BERTHIER, nat.SYS_NEWTONIAN, precision
-    p.rescale_temperature, p.cell_divisor = 0, cell_divisor
+    p.rescale_temperature, p.cell_divisor, p.build_mode = 0, cell_divisor, build_mode
     p.N, p.maxneighbours = N, 200
     p.box_size, p.force_cutoff, p.force_cutoff2, p.skin_distance = L, 1.25, 1.25 ** 2, 0.3
     p.b_c0, p.b_c2, p.b_c4, p.b_zeta = U.c0, U.c2, U.c4, U.ζ
@@ -210,7 +210,7 @@ def bench_large(args, nat, synth):
 
     log(f"[bench] setup N={N} L={L:.3f}")
     cc = None if args.no_copy_ceiling else nat.copy_bandwidth()  # streaming ceiling, 1 GiB copies
-    ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64, cell_divisor=args.cell_divisor)
+    ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64, cell_divisor=args.cell_divisor, build_mode=args.build_mode)
     ctx.set_state(r=r, v=np.zeros((N, dims)), F=np.zeros((N, dims)))
     ctx.update_neighbor_lists()
     t0 = time.perf_counter()
@@ -269,7 +269,7 @@ def bench_large(args, nat, synth):
                             "drift_window_steps": args.drift_steps})
 
     if not args.no_mixed:
-        ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED, cell_divisor=args.cell_divisor)
+        ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED, cell_divisor=args.cell_divisor, build_mode=args.build_mode)
         ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
         ctxm.update_neighbor_lists()
         ctx.set_state(r=state["r"], v=state["v"], F=state["F"])
@@ -314,7 +314,8 @@ def bench_large(args, nat, synth):
     return out
 
 
-def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300, 0.005, 0.02), retune=False):
+def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300, 0.005, 0.02), retune=False,
+                chunk=8):
     """Admit `specs` (cell divisor cd, build launch shape) into `streams`
     batches, relax, warm up, optionally re-tune the launch shape on the full
     batch, then time `steps` steps.  Returns the measurements."""
@@ -323,6 +324,7 @@ def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300
     else:
         b = nat.Batch(len(specs) + 1, graph=graph)
     b.set_build_launch(*shape)
+    b.set_ahead(chunk)
     ids, Ns, dims = [], [], []
     t0 = time.perf_counter()
     for s in specs:
@@ -368,10 +370,11 @@ def W_replica(spec, cd):
 def batched_summary(args, nat, steps=300, warmup=20):
     """The batched workload at a bounded length for the default run: the 256
     replicas at the launch choices the full autotuned run settles on (cell
-    divisor 1, build 256 threads x 16 lanes per slot, two streams)."""
+    divisor 1, build 256 threads x 4 lanes per slot, two streams, chunks of 8
+    steps)."""
     from simulationcode_amd import workloads as W
     t0 = time.perf_counter()
-    res = run_batched(nat, W.batched_specs(args.replicas), 1, (256, 16), 2, False, warmup, steps)
+    res = run_batched(nat, W.batched_specs(args.replicas), 1, (256, 4), 2, False, warmup, steps)
     res.update({"steps": steps, "warmup": warmup, "dt": 1e-4, "dynamics": "Brownian", "dtype": "mixed",
                 "cell_divisor": 1, "streams": 2, "wall_s": round(time.perf_counter() - t0, 1)})
     log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step, {res['value']:.3g} particle-steps/s, "
@@ -394,7 +397,7 @@ def bench_batched(args, nat):
     # on the full batch (the pilot holds a quarter of the replicas, so fewer
     # rebuild per step)
     res = run_batched(nat, specs, cd, shape, args.streams, args.graph, args.warmup, args.steps,
-                      retune=not args.cell_divisor)
+                      retune=not args.cell_divisor, chunk=args.chunk)
     log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step over {res['particles']} particles")
     return {"metric": METRIC, "value": res["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -434,6 +437,9 @@ def main():
     ap.add_argument("--cell-divisor", type=int, default=0,
                     help="0: library default (large) / autotuned (batched)")
     ap.add_argument("--shape", default="", help="batched with --cell-divisor: build launch shape BS,LPP")
+    ap.add_argument("--build-mode", type=int, default=0, help="large: sc_params.build_mode (0 automatic)")
+    ap.add_argument("--chunk", type=int, default=8,
+                    help="batched: steps launched ahead per chunk (0: one launch sequence per step)")
     ap.add_argument("--rate", type=float, nargs="*", default=None, help="served: arrival rates (jobs/s)")
     ap.add_argument("--jobs", type=int, default=48, help="served: jobs per arrival rate")
     args = ap.parse_args()

@@ -55,7 +55,8 @@ typedef struct sc_params {
     int32_t cell_divisor;         /* cells of side >= R_v / cell_divisor, stencil +-cell_divisor; 0 => 2 */
     int32_t build_mode;           /* 0: automatic (the per-slot sweep); 1: LDS-staged cell runs (cell_divisor 1);
                                      2: per-slot stencil sweep; 3: LDS windows with an FP32 prefilter
-                                     (cell_divisor 1-3); every mode writes the same lists */
+                                     (cell_divisor 1-3); 4: LDS-staged stencil rows, one wavefront per
+                                     64 slots; every mode writes the same lists */
     int32_t reserved;             /* must be 0 */
     int64_t N;
     int64_t maxneighbours;        /* initialize_neighbor_struct kwarg (200) */
@@ -238,12 +239,21 @@ int sc_batch_remove(sc_batch *b, int32_t id);
 /* Capture the step into a hipGraph (re-captured when the membership
  * changes) and replay it, instead of launching kernel by kernel. */
 int sc_batch_set_graph(sc_batch *b, int32_t on);
-/* Advance every admitted replica by nsteps steps. */
+/* Advance every admitted replica by nsteps steps.  By default the steps are
+ * enqueued ahead in chunks: a replica whose skin trigger fires halts for the
+ * rest of the chunk while the others go on, and the replicas that halted in a
+ * chunk are rebuilt together after it (one rebuild chain per chunk instead of
+ * one per step); every replica's trajectory is bitwise the one of eager
+ * stepping.  Batches with rescale_temperature replicas or a captured graph
+ * step eagerly. */
 int sc_batch_step(sc_batch *b, int64_t nsteps);
-/* sc_batch_step in two halves: enqueue the steps on the batch's stream and
- * return at once (several batches then run concurrently on their own
- * streams), and wait for them (device time, step counters, error check).
- * Any other call on the batch waits first. */
+/* Steps launched ahead per chunk (default 8), or 0 for one launch sequence
+ * per step; the results are the same. */
+int sc_batch_set_ahead(sc_batch *b, int32_t chunk);
+/* sc_batch_step in two halves: start the steps and return at once (several
+ * batches then run concurrently on their own streams; the chunked loop runs
+ * on a host thread of the library), and wait for them (device time, step
+ * counters, error check).  Any other call on the batch waits first. */
 int sc_batch_step_async(sc_batch *b, int64_t nsteps);
 int sc_batch_wait(sc_batch *b);
 /* Capped steepest descent (sc_minimize) of every admitted replica. */

@@ -97,7 +97,7 @@ double or_squared_distance(const double *ri, const double *rj, int dims, double 
     return s;
 }
 
-/* src/InteractionPotential.jl:107-117 */
+/* find_mean_D -- src/InteractionPotential.jl:1-11 (LJ :5-7, Berthier :9-11) */
 static inline double mean_d(const or_params *p, double Di, double Dj)
 {
     if (p->potential == OR_POT_LJ)
@@ -105,7 +105,7 @@ static inline double mean_d(const or_params *p, double Di, double Dj)
     return (Di + Dj) * 0.5 * (1.0 - p->b_zeta * fabs(Di - Dj));
 }
 
-/* src/InteractionPotential.jl:123-131 (LJ), :193-201 (Berthier) */
+/* potential -- src/InteractionPotential.jl:17-25 (LJ), :87-95 (Berthier) */
 static inline double potential(const or_params *p, double r2, double md2)
 {
     if (p->potential == OR_POT_LJ) {
@@ -121,7 +121,7 @@ static inline double potential(const or_params *p, double r2, double md2)
     return xi12 + p->b_c4 * invxi4 + p->b_c2 * invxi2 + p->b_c0;
 }
 
-/* src/InteractionPotential.jl:145-152 (LJ), :204-213 (Berthier) */
+/* force -- src/InteractionPotential.jl:39-46 (LJ), :98-107 (Berthier) */
 static inline double force_divr(const or_params *p, double r2, double md2)
 {
     if (p->potential == OR_POT_LJ) {

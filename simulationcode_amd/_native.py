@@ -50,7 +50,7 @@ EXPORTS = (
     "sc_batch_get_state", "sc_batch_set_state", "sc_batch_replica_stats", "sc_batch_info",
     "sc_batch_neighbor_stats", "sc_batch_replica_error",
     "sc_set_build_launch", "sc_batch_set_build_launch", "sc_autotune", "sc_batch_autotune",
-    "sc_batch_step_async", "sc_batch_wait",
+    "sc_batch_step_async", "sc_batch_wait", "sc_batch_set_ahead",
     "sc_set_reference_positions", "sc_f2_self", "sc_steinhardt",
 )
 
@@ -169,6 +169,7 @@ def load(path=LIB_PATH):
         "sc_batch_add": (ctypes.c_int, [vp, ctypes.POINTER(ScParams), vp, vp, vp, ctypes.POINTER(ctypes.c_int32)]),
         "sc_batch_remove": (ctypes.c_int, [vp, ctypes.c_int32]),
         "sc_batch_set_graph": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "sc_batch_set_ahead": (ctypes.c_int, [vp, ctypes.c_int32]),
         "sc_batch_step": (ctypes.c_int, [vp, i64]),
         "sc_batch_minimize": (ctypes.c_int, [vp, i64, dbl, dbl]),
         "sc_batch_minimize_until": (ctypes.c_int, [vp, i64, dbl, dbl, dbl, vp, vp]),
@@ -465,6 +466,12 @@ class Batch:
     def set_graph(self, on):
         self._check(self._lib.sc_batch_set_graph(self._h, 1 if on else 0))
 
+    def set_ahead(self, chunk):
+        """Steps launched ahead per chunk (True: the default 8), or False / 0
+        for eager per-step launches; the trajectories are the same."""
+        chunk = 8 if chunk is True else int(chunk)
+        self._check(self._lib.sc_batch_set_ahead(self._h, chunk))
+
     def step(self, nsteps):
         self._check(self._lib.sc_batch_step(self._h, int(nsteps)))
 
@@ -581,6 +588,9 @@ class BatchGroup:
 
     def set_build_launch(self, block_size, lanes):
         self._each("set_build_launch", block_size, lanes)
+
+    def set_ahead(self, chunk):
+        self._each("set_ahead", chunk)
 
     def minimize(self, iters, dt, max_step):
         self._each("minimize", iters, dt, max_step)

@@ -370,6 +370,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     c->geom.nruns = h.build_kind == BUILD_CELLS ? h.nruns : 0;
     c->geom.nwin = h.build_kind == BUILD_WIN ? h.nruns : 0;
     c->geom.any_tile_build = h.build_kind == BUILD_SLOT ? 1 : 0;
+    c->geom.any_rows_build = h.build_kind == BUILD_ROWS ? 1 << h.dims : 0;
     c->geom.flag = reinterpret_cast<const int *>(reinterpret_cast<const char *>(h.dyn) + offsetof(RepDyn, rebuild));
     rc = SC_OK;
     rc |= dalloc(c, &c->d_rep, 1);

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sc.h"
@@ -52,6 +53,11 @@ struct sc_batch {
     hipGraphExec_t gexec = nullptr;
     std::string err;
     double last_ms = 0.0;
+    int device = 0;
+    int chunk = 8;                // steps launched ahead per chunk (sc_batch_set_ahead; 0: eager)
+    int *h_state = nullptr;       // pinned: per-replica chunk state (k_gather_err what = 2)
+    std::thread worker;           // sc_batch_step_async's chunk loop, joined by sc_batch_wait
+    int worker_rc = SC_OK;
 };
 
 namespace {
@@ -112,6 +118,7 @@ int sync_tables(sc_batch *b)
         reps.push_back(sl.h);
         b->geom.max_chunks = std::max(b->geom.max_chunks, (sl.h.ncell + SC_CHUNK - 1) / SC_CHUNK);
         if (sl.h.build_kind == BUILD_SLOT) b->geom.any_tile_build = 1;
+        if (sl.h.build_kind == BUILD_ROWS) b->geom.any_rows_build |= 1 << sl.h.dims;
         if (sl.h.sys == SC_SYS_BROWNIAN)
             b->has_bd = true;
         else
@@ -161,6 +168,69 @@ int enqueue_step(sc_batch *b)
     return SC_OK;
 }
 
+// Steps enqueued ahead in chunks, as a single system does (sc_api.hip), for
+// every replica at once.  Within a chunk each replica runs its own steps
+// until its drift fires its skin trigger (the replica then halts: its later
+// kernels of the chunk return at once, RepDyn::halt) or it reaches this
+// call's step target; the others go on.  After each chunk one copy reads
+// every replica's state, and the halted replicas are rebuilt together in one
+// rebuild chain (velocity Verlet: followed by the force and finish of the
+// halted step, for those replicas only), then the next chunk continues.
+// Each replica's kernels and their order are those of enqueue_step, so its
+// trajectory is bitwise the same; the rebuild work of a chunk is shared by
+// all replicas that triggered in it instead of a rebuild chain per step.
+int steps_ahead_batch(sc_batch *b, int64_t nsteps)
+{
+    const int nt = b->ntiles, nr = b->nact;
+    BK(b, sc::launch_set_target(b->d_reps, nr, nsteps, b->s));
+    int64_t left = nsteps;
+    while (left > 0) {
+        const int64_t m = std::min<int64_t>(b->chunk, left);
+        for (int64_t i = 0; i < m; i++) {
+            if (b->has_bd) {
+                BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 2, 4, b->s, false, true));
+                BK(b, sc::launch_force_finish(b->d_reps, nr, 2, 4, b->s, true));
+                BK(b, sc::launch_drift_brownian(b->d_tiles, nt, b->d_reps, nr, b->s, true));
+            }
+            if (b->has_vv) {
+                BK(b, sc::launch_kick_drift_only(b->d_tiles, nt, b->d_reps, b->s, true));
+                BK(b, sc::launch_force(b->d_tiles, nt, b->d_reps, nr, 1, 3, b->s, false, true));
+                BK(b, sc::launch_force_finish(b->d_reps, nr, 1, 3, b->s, true));
+            }
+        }
+        BK(b, sc::launch_gather_err(b->d_reps, nr, b->d_err, b->s, 2));
+        BCHK(b, hipMemcpyAsync(b->h_state, b->d_err, sizeof(int) * nr, hipMemcpyDeviceToHost, b->s));
+        BCHK(b, hipStreamSynchronize(b->s));
+        bool halted = false;
+        int64_t most = 0;
+        for (int r = 0; r < nr; r++) {
+            const int st = b->h_state[r];
+            int64_t l = st >> 2;
+            if (st & 3) {
+                halted = true;
+                // a halted velocity-Verlet replica completes its halted step below
+                if (b->slots[b->act[r]].h.sys != SC_SYS_BROWNIAN && (st & 1)) l -= 1;
+            }
+            most = std::max(most, l);
+        }
+        if (halted) {
+            BK(b, sc::launch_rebuild(b->d_tiles, nt, b->d_reps, nr, b->geom, b->s, false));
+            if (b->has_vv) BK(b, sc::launch_force_resume(b->d_tiles, nt, b->d_reps, nr, b->s));
+            if (b->has_bd) BK(b, sc::launch_rebuild_done(b->d_reps, nr, b->s, 4));
+        }
+        // every replica short of its target advances at least one step per
+        // chunk (a Brownian replica halts only after the step whose drift
+        // fired, a velocity-Verlet one completes that step above)
+        if (most >= left) {
+            BK(b, sc::launch_set_target(b->d_reps, nr, -1, b->s));
+            return bfail(b, SC_ERR_STATE, "chunked steps made no progress");
+        }
+        left = most;
+    }
+    BK(b, sc::launch_set_target(b->d_reps, nr, -1, b->s));
+    return SC_OK;
+}
+
 int check_slot(sc_batch *b, int id)
 {
     if (!b || id < 0 || id >= (int)b->slots.size() || !b->slots[id].used) return SC_ERR_ARG;
@@ -202,6 +272,11 @@ int sc_batch_create(sc_batch **out, int32_t max_replicas)
     b->max_reps = max_replicas;
     b->slots.resize(max_replicas);
     if (hipStreamCreate(&b->s) != hipSuccess) return bfail(b, SC_ERR_HIP, "hipStreamCreate failed");
+    BCHK(b, hipGetDevice(&b->device));
+    if (hipHostMalloc((void **)&b->h_state, sizeof(int) * max_replicas, hipHostMallocDefault) != hipSuccess) {
+        b->h_state = nullptr;
+        return bfail(b, SC_ERR_OOM, "pinned chunk-state buffer");
+    }
     BCHK(b, hipEventCreate(&b->ev0));
     BCHK(b, hipEventCreate(&b->ev1));
     if (hipMalloc(&b->d_reps, sizeof(Rep) * max_replicas) != hipSuccess)
@@ -214,7 +289,9 @@ int sc_batch_create(sc_batch **out, int32_t max_replicas)
 void sc_batch_destroy(sc_batch *b)
 {
     if (!b) return;
+    if (b->worker.joinable()) b->worker.join();
     if (b->s) (void)hipStreamSynchronize(b->s);
+    if (b->h_state) (void)hipHostFree(b->h_state);
     drop_graph(b);
     for (auto &sl : b->slots) sc::free_all(sl.allocs);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
@@ -301,6 +378,17 @@ int sc_batch_remove(sc_batch *b, int32_t id)
     return SC_OK;
 }
 
+int sc_batch_set_ahead(sc_batch *b, int32_t chunk)
+{
+    if (b && b->pending >= 0) {
+        int rc = sc_batch_wait(b);
+        if (rc) return rc;
+    }
+    if (!b || chunk < 0 || chunk > 4096) return SC_ERR_ARG;
+    b->chunk = chunk;
+    return SC_OK;
+}
+
 int sc_batch_set_graph(sc_batch *b, int32_t on)
 {
     if (b && b->pending >= 0) {
@@ -324,6 +412,21 @@ int sc_batch_step_async(sc_batch *b, int64_t nsteps)
         return SC_OK;
     }
     BCHK(b, hipEventRecord(b->ev0, b->s));
+    if (b->chunk > 0 && !b->use_graph && !b->has_rescale) {
+        // the chunk loop polls the device between chunks: it runs on a worker
+        // thread (on the batch's device and stream) so this call returns at once
+        b->worker_rc = SC_OK;
+        b->pending = nsteps;
+        b->worker = std::thread([b, nsteps]() {
+            int rc2 = hipSetDevice(b->device) == hipSuccess ? SC_OK : bfail(b, SC_ERR_HIP, "hipSetDevice failed");
+            if (!rc2) rc2 = steps_ahead_batch(b, nsteps);
+            if (!rc2 && sc::launch_finalize(b->d_tiles, b->ntiles, b->d_reps, b->nact, b->s))
+                rc2 = bfail(b, SC_ERR_HIP, "kernel launch failed (finalize)");
+            if (!rc2 && hipEventRecord(b->ev1, b->s) != hipSuccess) rc2 = bfail(b, SC_ERR_HIP, "hipEventRecord failed");
+            b->worker_rc = rc2;
+        });
+        return SC_OK;
+    }
     if (b->use_graph) {
         if (!b->gexec) {
             BCHK(b, hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
@@ -353,6 +456,10 @@ int sc_batch_wait(sc_batch *b)
     if (b->pending < 0) return SC_OK;
     const int64_t n = b->pending;
     b->pending = -1;
+    if (b->worker.joinable()) {
+        b->worker.join();
+        if (b->worker_rc) return b->worker_rc;
+    }
     if (n == 0) return SC_OK;
     BCHK(b, hipEventSynchronize(b->ev1));
     float ms = 0.f;

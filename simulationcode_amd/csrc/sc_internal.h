@@ -33,7 +33,7 @@
 #define SC_WIN_ROWS 64                  // own cell rows of one window run
 
 // neighbour-list build kinds of a replica
-enum { BUILD_SLOT = 0, BUILD_CELLS = 1, BUILD_WIN = 2 };
+enum { BUILD_SLOT = 0, BUILD_CELLS = 1, BUILD_WIN = 2, BUILD_ROWS = 3 };
 
 #define REBUILD_KEEP_F 2
 
@@ -63,6 +63,8 @@ struct RepDyn {
                                       // trigger, so its force and every later kernel of the chunk
                                       // return at once until the host has run the rebuild
     long long min_iters;              // iterations done by the current tolerance run
+    long long target;                 // steps launched ahead for a batch stop at this step count
+                                      // (LLONG_MAX outside such a call)
     double vmean[3];
     double vscale;
     double Fmean[3];
@@ -80,7 +82,8 @@ struct Rep {
     int rescale;
     int ntiles, tile0;   // tiles of this replica in the launch table
     int build_kind;      // BUILD_SLOT: per-slot sweep; BUILD_CELLS: LDS cell runs (stencil +-1);
-                         // BUILD_WIN: LDS windows over cx x cy x cz own cells (any stencil)
+                         // BUILD_WIN: LDS windows over cx x cy x cz own cells (any stencil);
+                         // BUILD_ROWS: LDS-staged stencil rows, one wave per 64 slots
     int cx, cy, cz;      // own cells per build run along x, y, z
     int nruns, run0;     // build runs of this replica and their offset in the launch
     float wlo, whi;      // windowed build: FP32 d^2 below wlo is certainly <= R_v^2, above whi
@@ -145,6 +148,7 @@ struct RebuildGeom {
     int nruns = 0;           // LDS cell-run build runs over all replicas (BUILD_CELLS)
     int nwin = 0;            // windowed-build runs over all replicas (BUILD_WIN)
     int any_tile_build = 0;  // some replica uses the per-slot build (all-pairs / explicit)
+    int any_rows_build = 0;  // replicas use the staged-row build: bit 2 some 2D replica, bit 3 some 3D one
     int *plan = nullptr;     // device work list, plan_ints(nreps, ntiles, nwin) ints (k_rebuild_plan)
     const int *flag = nullptr;  // one replica without a plan: its RepDyn::rebuild on the device
     int build_bs = 256;      // per-slot build launch shape: threads per block (64 or 256)
@@ -162,9 +166,16 @@ constexpr int kBuildShapes[12][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {25
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
                    bool bookkeep = true);
 int launch_request_rebuild(const Rep *reps, int nreps, hipStream_t s);
-int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s);  // rebuild bookkeeping alone
-// per replica: what = 0 its error bits, 1 its minimiser stop state
+// rebuild bookkeeping alone (replicas of the dynamics in sysmask: bit 0 Newtonian, 1 Langevin, 2 Brownian)
+int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s, int sysmask = 7);
+// per replica: what = 0 its error bits, 1 its minimiser stop state, 2 its
+// chunk state (bit 0 halt, bit 1 rebuild flag, bits 2.. steps left to its target)
 int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s, int what = 0);
+// step targets of a batch call launched ahead: target = step + nsteps (nsteps >= 0), or none (< 0)
+int launch_set_target(const Rep *reps, int nreps, long long nsteps, hipStream_t s);
+// after a chunk launched ahead: the force (mode 1) and finish of the step whose
+// drift halted each halted velocity-Verlet replica, for those replicas only
+int launch_force_resume(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 int launch_finalize(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s);
 // per (optional): E_i / 2 of every particle at its original index
 int launch_energy(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s, double *per = nullptr);

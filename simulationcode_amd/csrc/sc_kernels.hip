@@ -16,6 +16,7 @@
 // launch sequence can be replayed without host work.
 #include "sc_internal.h"
 
+#include <climits>
 #include <math.h>
 
 // Device buffers are reached through the pointers of a Rep read from memory,
@@ -248,7 +249,7 @@ __device__ inline double rcp64(double x)
 
 // FP64 force/r of one pair, the formulas of the reference loops:
 // Berthier src/MD.jl:120-126 (3D) / :228-234 (2D); LJ 3D :175-180; generic
-// (2D LJ) :299-307 with force() of src/InteractionPotential.jl:145-152.  The
+// (2D LJ) :299-307 with force() of src/InteractionPotential.jl:39-46.  The
 // cutoff tests are evaluated exactly as there; divisions are reciprocals
 // (rcp64), which moves the force by a few ulp.
 template <int DIMS, int POT>
@@ -374,8 +375,9 @@ __device__ inline double pair_energy(const Rep &R, double rij2, double Di, doubl
 }
 
 // Potential of calculate_full_energy_no_neigh (generic potential(),
-// src/AuxiliaryFunctions.jl:94-108 with src/InteractionPotential.jl:123-131,
-// :193-201): mean_d from find_mean_D for every potential.
+// src/AuxiliaryFunctions.jl:94-108 with potential() of
+// src/InteractionPotential.jl:17-25 (LJ) and :87-95 (Berthier)): mean_d from
+// find_mean_D (:1-11) for every potential.
 template <int POT>
 __device__ inline double pair_energy_generic(const Rep &R, double rij2, double Di, double Dj)
 {
@@ -497,8 +499,8 @@ __global__ __launch_bounds__(SC_TB) void k_kick_drift(const Tile *tiles, const R
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     if (R.sys == 2) return;
-    // halt is written only by the force finish, never during this launch
-    if (ahead && gp(R.dyn)->halt) return;
+    // halt (and step) are written only by the force finish, never during this launch
+    if (ahead && (gp(R.dyn)->halt || gp(R.dyn)->step >= gp(R.dyn)->target)) return;
     if (R.dims == 3)
         kick_drift_tile<3>(R, t);
     else
@@ -547,7 +549,9 @@ __global__ __launch_bounds__(SC_TB) void k_drift_brownian(const Tile *tiles, con
     const Tile t = tiles[xcd_tile()];
     const Rep R = reps[t.rep];
     if (R.sys != 2) return;
-    if (ahead && gp(R.dyn)->halt) return;  // halt is written only by the force finish
+    // halt is written only by the force finish: bit 0 the chunk halted, bit 1
+    // the replica has done its steps (this step's force finish saw it)
+    if (ahead && gp(R.dyn)->halt) return;
     if (R.dims == 3)
         drift_brownian_tile<3>(R, t);
     else
@@ -689,8 +693,10 @@ __global__ __launch_bounds__(SC_TB) void k_force(const Tile *tiles, const Rep *r
     const Rep R = reps[t.rep];
     if (!((sysmask >> R.sys) & 1)) return;
     // launched ahead: this step's drift fired the trigger (the rebuild must
-    // come first) or an earlier one did (the chunk has halted)
-    if (ahead && (gp(R.dyn)->halt || gp(R.dyn)->rebuild)) return;
+    // come first), an earlier one did (the chunk has halted) or the replica
+    // has done its steps; ahead == 2: the resumed step of a halted replica only
+    if (ahead == 1 && (gp(R.dyn)->halt || gp(R.dyn)->rebuild || gp(R.dyn)->step >= gp(R.dyn)->target)) return;
+    if (ahead == 2 && !(gp(R.dyn)->halt & 1)) return;
     if (R.min_tol >= 0.0 && gp(R.dyn)->min_state != 0) return;  // minimiser converged
     const int key = (R.dims == 3 ? 4 : 0) + (R.pot == 2 ? 2 : 0) + (R.prec ? 1 : 0);
     switch (key) {
@@ -781,7 +787,13 @@ __global__ void k_gather_err(const Rep *reps, int nreps, int *out, int what)
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < nreps) {
         const SC_AS1 RepDyn *d = gp(reps[r].dyn);
-        out[r] = what == 0 ? d->err : d->min_state;
+        if (what == 2) {
+            const long long left = d->target - d->step;
+            const int l = left <= 0 ? 0 : (left > (1 << 28) ? (1 << 28) : (int)left);
+            out[r] = (d->halt & 1) | (d->rebuild ? 2 : 0) | (l << 2);
+        } else {
+            out[r] = what == 0 ? d->err : d->min_state;
+        }
     }
 }
 
@@ -1495,7 +1507,7 @@ __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long 
 // block, LPP lanes per slot; a work item is BS/LPP consecutive slots of one
 // tile.  The lists do not depend on the launch shape.  With one lane per
 // slot the register budget is held to five wavefronts per SIMD (96 VGPRs,
-// no spills).
+// ten of them spilled to scratch).
 template <int BS, int LPP>
 __global__ __launch_bounds__(BS, LPP == 1 ? 5 : 1) void k_build(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
                                               int nreps, int ntiles)
@@ -1524,6 +1536,172 @@ __global__ __launch_bounds__(BS, LPP == 1 ? 5 : 1) void k_build(const Tile *tile
             else
                 build_slot_lanes<2, LPP>(R, k, l, gmask, below);
         }
+    }
+}
+
+// R5 (staged-row form, stencil +-s, s >= 1).  One wave (a block of 64
+// threads) per 64 consecutive slots.  Its lanes group by own cell row (y, z)
+// -- cell-sorted slots span one row, sometimes two -- and for each of the
+// group's (2s+1)^(dims-1) stencil rows the wave stages the candidates of the
+// whole raw x range [xmin - s, xmax + s] into LDS with coalesced loads (one
+// contiguous slot range per wrap part), with the staged offset of every raw
+// cell beside them.  Each lane then sweeps its own x window (cut to the list
+// sphere as in for_stencil_rows) from LDS.  Candidates keep the per-slot
+// sweep's order (rows in order, raw x ascending, slots ascending) and the
+// same exact image-shift d^2 (build_one_img), so the lists are entry for
+// entry those of the per-slot sweep; only where the candidates come from
+// changes.  Rows longer than RB_CAP candidates are staged in chunks.
+constexpr int RB_CAP = 256;   // staged candidates per chunk
+constexpr int RB_XMAX = 192;  // raw cells whose offsets fit the LDS table
+
+struct RowsLds {
+    double4 q[RB_CAP];
+    int off[RB_XMAX + 1];
+};
+
+__device__ inline int wave_min_int(int x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, WAVE));
+    return x;
+}
+
+__device__ inline int wave_max_int(int x)
+{
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, WAVE));
+    return x;
+}
+
+// Staged offset of raw cell x (rlo <= x <= rhi + 1) of stencil row `row`: the
+// raw range is laid out as wrap part A (x < 0, cells x + nc), part B
+// (0 <= x < nc) and part C (x >= nc, cells x - nc), each one slot range.
+struct RowParts {
+    int sA, nA, sB, nB, sC, nC;
+};
+
+__device__ inline int staged_offset(const Rep &R, int row, int x, int rhi, const RowParts &P)
+{
+    if (x > rhi) return P.nA + P.nB + P.nC;
+    if (x < 0) return cstart(R, row + x + R.nc) - P.sA;
+    if (x < R.nc) return P.nA + cstart(R, row + x) - P.sB;
+    return P.nA + P.nB + cstart(R, row + x - R.nc) - P.sC;
+}
+
+template <int DIMS>
+__device__ void rows_group(const Rep &R, int k0, RowsLds &S)
+{
+    const int lane = threadIdx.x, n = R.n, nc = R.nc, s = R.srange;
+    const int k = k0 + lane;
+    const bool valid = k < n;
+    const double4 p = valid ? ld4(gp(R.pos_alt) + (k)) : make_double4(0.0, 0.0, 0.0, 0.0);
+    int cc[3];
+    cell_of_pos<DIMS>(R, p, cc);
+    const double fx = p.x * R.cinv - (double)cc[0], fy = p.y * R.cinv - (double)cc[1];
+    const double fz = DIMS == 3 ? p.z * R.cinv - (double)cc[2] : 0.0;
+    const double rc2 = R.Rv2 * R.cinv * R.cinv * (1.0 + 1e-9) + 1e-9;  // R_v^2 in cell units
+    const int zr = DIMS == 3 ? s : 0;
+    int nfull = 0, cnt = 0;
+    unsigned long long todo = __ballot(valid);
+    while (todo) {  // the distinct own rows of the group (usually one)
+        // the row loop's values are wave-uniform: readfirstlane keeps them in
+        // scalar registers (and the row loop's branches scalar)
+        const int leader = __ffsll((long long)todo) - 1;
+        const int ry = __builtin_amdgcn_readfirstlane(__shfl(cc[1], leader, WAVE));
+        const int rz = __builtin_amdgcn_readfirstlane(__shfl(cc[2], leader, WAVE));
+        const bool mine = valid && cc[1] == ry && cc[2] == rz;
+        todo &= ~__ballot(mine);
+        const int rlo = __builtin_amdgcn_readfirstlane(wave_min_int(mine ? cc[0] : 1 << 30)) - s;
+        const int rhi = __builtin_amdgcn_readfirstlane(wave_max_int(mine ? cc[0] : -(1 << 30))) + s;
+        const int ncells = rhi - rlo + 2;  // raw cells plus the end
+        for (int oz = -zr; oz <= zr; oz++) {
+            double sz = 0.0;
+            const int cz = DIMS == 3 ? wrap_shift(rz + oz, nc, R.L, sz) : 0;
+            const double gz = DIMS == 3 ? cell_gap64(oz, fz) : 0.0;
+            for (int oy = -s; oy <= s; oy++) {
+                double sy;
+                const int cy = wrap_shift(ry + oy, nc, R.L, sy);
+                const int row = nc * (cy + nc * cz);
+                RowParts P;
+                {
+                    const int a1 = min(rhi, -1), b0 = max(rlo, 0), b1 = min(rhi, nc - 1), c0 = max(rlo, nc);
+                    P.sA = rlo < 0 ? cstart(R, row + rlo + nc) : 0;
+                    P.nA = rlo < 0 ? cstart(R, row + a1 + nc + 1) - P.sA : 0;
+                    P.sB = b0 <= b1 ? cstart(R, row + b0) : 0;
+                    P.nB = b0 <= b1 ? cstart(R, row + b1 + 1) - P.sB : 0;
+                    P.sC = rhi >= nc ? cstart(R, row + c0 - nc) : 0;
+                    P.nC = rhi >= nc ? cstart(R, row + rhi - nc + 1) - P.sC : 0;
+                }
+                const int T = P.nA + P.nB + P.nC;
+                // this lane's x window, cut to the list sphere, in raw cells
+                int lo = 1, hi = 0;
+                if (mine) {
+                    const double gy = cell_gap64(oy, fy);
+                    const double g2 = gy * gy + gz * gz;
+                    if (g2 <= rc2) {
+                        const double rx = sqrt(rc2 - g2) + 1e-9;
+                        lo = cc[0] + max(-s, (int)floor(fx - rx));
+                        hi = cc[0] + min(s, (int)floor(fx + rx));
+                    }
+                }
+                int a0 = 0, a1 = 0;
+                const bool tab = ncells <= RB_XMAX + 1;
+                __syncthreads();  // the previous row's readers are done with the LDS
+                if (tab)
+                    for (int x = lane; x < ncells; x += WAVE) S.off[x] = staged_offset(R, row, rlo + x, rhi, P);
+                __syncthreads();
+                if (lo <= hi) {
+                    a0 = tab ? S.off[lo - rlo] : staged_offset(R, row, lo, rhi, P);
+                    a1 = tab ? S.off[hi + 1 - rlo] : staged_offset(R, row, hi + 1, rhi, P);
+                }
+                for (int c0 = 0; c0 < T; c0 += RB_CAP) {
+                    const int cn = min(RB_CAP, T - c0);
+                    if (c0 > 0) __syncthreads();
+                    for (int a = lane; a < cn; a += WAVE) {
+                        const int g = c0 + a;
+                        const int j = g < P.nA ? P.sA + g : (g < P.nA + P.nB ? P.sB + (g - P.nA) : P.sC + (g - P.nA - P.nB));
+                        S.q[a] = ld4(gp(R.pos_alt) + (j));
+                    }
+                    __syncthreads();
+                    const int b0 = max(a0, c0), b1 = min(a1, c0 + cn);
+#pragma unroll 1
+                    for (int g = b0; g < b1; g++) {
+                        int j;
+                        double sx;
+                        if (g < P.nA) {
+                            j = P.sA + g;
+                            sx = -R.L;
+                        } else if (g < P.nA + P.nB) {
+                            j = P.sB + (g - P.nA);
+                            sx = 0.0;
+                        } else {
+                            j = P.sC + (g - P.nA - P.nB);
+                            sx = R.L;
+                        }
+                        if (j != k) build_one_img<DIMS>(R, k, p, nfull, cnt, j, S.q[g - c0], sx, sy, sz);
+                    }
+                }
+            }
+        }
+    }
+    if (valid) build_finish_slot<DIMS>(R, k, p, nfull, cnt);
+}
+
+// One kernel per dimensionality (each holds only its own sweep's registers).
+template <int DIMS>
+__global__ __launch_bounds__(WAVE) void k_build_rows(const Tile *tiles, const Rep *reps, const int *plan,
+                                                     const int *flag, int nreps, int ntiles)
+{
+    __shared__ RowsLds S;
+    constexpr int GPT = SC_TB / WAVE;  // 64-slot groups per tile
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
+    for (XcdRange it = xcd_range(wk.count * GPT); it.cur < it.end; it.cur += it.step) {
+        const Tile t = tiles[wk.tile(it.cur / GPT)];
+        const Rep R = reps[t.rep];
+        const int k0 = t.first + (it.cur % GPT) * WAVE;
+        // uniform over the block (one wave)
+        if (R.build_kind != BUILD_ROWS || R.dims != DIMS || k0 >= R.n) continue;
+        rows_group<DIMS>(R, k0, S);
     }
 }
 
@@ -2547,7 +2725,9 @@ enum {
     FIN_MIN_END,        // ... and leave the force kernels unblocked
     FIN_MEAN,           // compensated total / n
     FIN_FORCE1_AHEAD,   // FIN_FORCE1 of a step launched ahead: halts the chunk instead when the trigger fired
-    FIN_FORCE2_AHEAD    // the same for FIN_FORCE2
+    FIN_FORCE2_AHEAD,   // the same for FIN_FORCE2
+    FIN_FORCE1_RESUME,  // FIN_FORCE1 of the resumed step of a halted replica (others: nothing)
+    FIN_SET_TARGET      // target = step + arg (arg >= 0), else none
 };
 
 __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, int op, int sysmask, double arg)
@@ -2561,12 +2741,31 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     const SC_AS1 double *P = gp(R.part);
     if (op == FIN_FORCE1_AHEAD || op == FIN_FORCE2_AHEAD) {
         const int h = dyn->halt, rb = dyn->rebuild;
+        const bool done = dyn->step >= dyn->target;
         __syncthreads();  // every thread has read the flags before thread 0 may set halt
+        if (done) {
+            // the replica's steps of this call are done (its force kernel
+            // returned too): bit 1 stops the rest of this step (the Brownian
+            // drift) and every later step of the call
+            if (tid == 0) dyn->halt = h | 2;
+            return;
+        }
         if (h || rb) {
             if (!h && tid == 0) dyn->halt = 1;  // the host runs the rebuild and this step again
             return;
         }
         op = op == FIN_FORCE1_AHEAD ? FIN_FORCE1 : FIN_FORCE2;
+    } else if (op == FIN_FORCE1_RESUME) {
+        const int h = dyn->halt;
+        __syncthreads();  // read before FIN_FORCE1 clears it
+        if (!(h & 1)) return;
+        op = FIN_FORCE1;
+    } else if (op == FIN_SET_TARGET) {
+        if (tid == 0) {
+            dyn->target = arg >= 0.0 ? dyn->step + (long long)arg : LLONG_MAX;
+            dyn->halt &= 1;  // a finished call's done bit
+        }
+        return;
     }
     switch (op) {
     case FIN_DRIFT:
@@ -2758,6 +2957,20 @@ int launch_force_finish(const Rep *reps, int nreps, int mode, int sysmask, hipSt
     return check(hipGetLastError());
 }
 
+int launch_set_target(const Rep *reps, int nreps, long long nsteps, hipStream_t s)
+{
+    fin(reps, nreps, FIN_SET_TARGET, SYS_ALL, nsteps >= 0 ? (double)nsteps : -1.0, s);
+    return check(hipGetLastError());
+}
+
+int launch_force_resume(const Tile *tiles, int ntiles, const Rep *reps, int nreps, hipStream_t s)
+{
+    if (ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_force<1>, dim3(ntiles), dim3(SC_TB), 0, s, tiles, reps, 3, 2);
+    fin(reps, nreps, FIN_FORCE1_RESUME, 3, 0.0, s);
+    return check(hipGetLastError());
+}
+
 int launch_kick_drift_only(const Tile *tiles, int ntiles, const Rep *reps, hipStream_t s, bool ahead)
 {
     if (ntiles <= 0) return 0;
@@ -2780,9 +2993,9 @@ int launch_ekin_com(const Tile *tiles, int ntiles, const Rep *reps, int nreps, h
     return check(hipGetLastError());
 }
 
-int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s)
+int launch_rebuild_done(const Rep *reps, int nreps, hipStream_t s, int sysmask)
 {
-    fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);
+    fin(reps, nreps, FIN_REBUILD_DONE, sysmask, 0.0, s);
     return check(hipGetLastError());
 }
 
@@ -2838,6 +3051,14 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     if (g.nwin > 0) {
         const int gw = g.nwin < SC_RUN_GRID_CAP ? g.nwin : SC_RUN_GRID_CAP;
         hipLaunchKernelGGL(k_build_win, dim3(gw), dim3(SC_TB), 0, s, reps, plan, flag, nreps, ntiles);
+    }
+    if (g.any_rows_build) {
+        const int items = ntiles * (SC_TB / 64), cap = 4 * SC_GRID_CAP;
+        const dim3 grid(items < cap ? items : cap);
+        if (g.any_rows_build & 8)
+            hipLaunchKernelGGL(k_build_rows<3>, grid, dim3(64), 0, s, tiles, reps, plan, flag, nreps, ntiles);
+        if (g.any_rows_build & 4)
+            hipLaunchKernelGGL(k_build_rows<2>, grid, dim3(64), 0, s, tiles, reps, plan, flag, nreps, ntiles);
     }
     if (g.any_tile_build) {
         const int key = g.build_bs * 64 + g.build_lpp;

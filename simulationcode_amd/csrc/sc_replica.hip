@@ -5,6 +5,7 @@
 // radius R_v = skin + force_cutoff * max(D); the cell grid, the tile count and
 // every device buffer follow from it and from N.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -21,7 +22,7 @@ bool params_valid(const sc_params *p)
              p->system < 0 || p->system > 2 || (p->precision != SC_PREC_FP64 && p->precision != SC_PREC_MIXED) ||
              p->N < 2 || p->N > (int64_t)1 << 30 || p->maxneighbours < 1 || p->maxneighbours > 100000 ||
              !(p->box_size > 0.0) || !(p->skin_distance >= 0.0) || !(p->force_cutoff > 0.0) ||
-             p->cell_divisor < 0 || p->cell_divisor > 4 || p->build_mode < 0 || p->build_mode > 3);
+             p->cell_divisor < 0 || p->cell_divisor > 4 || p->build_mode < 0 || p->build_mode > 4);
 }
 
 // Cell grid of a replica: cells of side >= R_v / k with a (2k+1)^dims
@@ -202,8 +203,9 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     h.cinv = (double)nc / p->box_size;
     h.ntiles = (n + SC_TB - 1) / SC_TB;
     // build kind: the windowed LDS build (build_mode 3, stencils up to +-3),
-    // the cell-run LDS build (build_mode 1, stencil +-1), else (automatic and
-    // build_mode 2) the per-slot sweep
+    // the cell-run LDS build (build_mode 1, stencil +-1), the staged-row LDS
+    // build (build_mode 4, any stencil), else (automatic and build_mode 2) the
+    // per-slot sweep
     h.wcell = p->box_size / (double)nc;
     h.dmax = (float)maxD;
     // the windowed build's FP32 half-list radius stays within its margin while
@@ -223,6 +225,8 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
         h.cx = cx;
         const long long rpr = (nc + cx - 1) / cx;
         h.nruns = (int)(rpr * nc * (p->dims == 3 ? nc : 1));
+    } else if (srange >= 1 && p->build_mode == 4) {
+        h.build_kind = BUILD_ROWS;
     } else {
         h.build_kind = BUILD_SLOT;
     }
@@ -260,6 +264,7 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     RepDyn dyn;
     std::memset(&dyn, 0, sizeof(dyn));
     dyn.vscale = 1.0;
+    dyn.target = LLONG_MAX;
     bool ok = true;
     ok &= hipMemcpy(h.orig, ident.data(), sizeof(int) * n, hipMemcpyHostToDevice) == hipSuccess;
     ok &= hipMemcpy(h.pos, pos.data(), sizeof(double4) * n, hipMemcpyHostToDevice) == hipSuccess;

@@ -133,3 +133,29 @@ def test_stream_group_and_async_match_single_batch():
     assert st["steps"] == 80
     assert np.array_equal(b.get_state(rid)["r"], ref[0]["r"])
     b.close()
+
+
+def test_chunked_ahead_equals_eager_stepping():
+    """Steps enqueued ahead in chunks (replicas halting on their own skin
+    triggers, rebuilt together after each chunk) give bitwise the results of
+    one launch sequence per step, for every replica of a mix of dynamics,
+    dimensions and precisions, over call lengths that end mid-chunk; rebuild
+    counts agree."""
+    specs = SPECS + [dict(N=3000, dims=3, system="brownian", kBT=0.3, dt=0.0005, seed=6, precision="mixed"),
+                     dict(N=1200, dims=2, system="newtonian", kBT=0.3, dt=0.006, seed=7, precision="fp64")]
+    out = []
+    for ahead in (False, True):
+        b = nat.Batch(16)
+        b.set_ahead(ahead)
+        ids = [b.add(*W.replica(**s)) for s in specs]
+        b.minimize(*RELAX)
+        for k in (37, 1, 100, 16, 201):
+            b.step(k)
+        out.append(([b.get_state(i) for i in ids], [b.stats(i) for i in ids]))
+        b.close()
+    (se, te), (sa, ta) = out
+    for k in range(len(specs)):
+        for f in ("r", "v", "F"):
+            assert np.array_equal(se[k][f], sa[k][f]), (k, f)
+        assert te[k]["rebuilds"] == ta[k]["rebuilds"] and te[k]["steps"] == ta[k]["steps"] == 355
+    assert sum(t["rebuilds"] for t in ta) > 3 * len(specs)

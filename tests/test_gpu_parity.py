@@ -33,7 +33,7 @@ def case_id(c):
 # build_mode 0: automatic; 1: LDS cell runs (stencil +-1); 2: per-slot sweep
 # over global memory; 3: LDS windows (FP32 prefilter, FP64 band test)
 @pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (3, 0), (1, 1), (1, 2), (2, 2),
-                                                     (1, 3), (2, 3), (3, 3)])
+                                                     (1, 3), (2, 3), (3, 3), (1, 4), (2, 4), (3, 4)])
 def test_pair_sets_bit_exact(case, cell_divisor, build_mode):
     kind, dims, N = case
     op, r, D, v, rc = H.make_system(kind, N, dims, seed=100 + N)
@@ -74,7 +74,7 @@ def _threshold_lattice(dims, poly, jitter, seed):
 @pytest.mark.parametrize("dims", [3, 2])
 @pytest.mark.parametrize("poly", [False, True])
 @pytest.mark.parametrize("jitter", [0.0, 1e-7, 1e-5, 3e-4])
-@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (1, 3)])
+@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 0), (2, 0), (1, 3), (2, 4)])
 def test_pair_sets_at_exact_thresholds(dims, poly, jitter, cell_divisor, build_mode):
     op, r, D = _threshold_lattice(dims, poly, jitter, seed=dims * 10 + int(poly))
     ctx = make_ctx(op, 1.25, r, np.zeros_like(r), D, cell_divisor=cell_divisor, build_mode=build_mode)
@@ -201,7 +201,7 @@ def test_trajectory_rescale_temperature():
     assert np.max(np.abs(g["v"] - st.v)) <= 1e-10
 
 
-@pytest.mark.parametrize("cell_divisor,build_mode", [(2, 0), (1, 1), (2, 3)])
+@pytest.mark.parametrize("cell_divisor,build_mode", [(2, 0), (1, 1), (2, 3), (2, 4), (3, 4)])
 def test_trajectory_many_rebuilds(cell_divisor, build_mode):
     op, st, g, dr, ctx = _trajectory("berthier", 3, 1000, oracle.SYS_LANGEVIN, 400, dt=0.01,
                                       cell_divisor=cell_divisor, build_mode=build_mode)
@@ -225,6 +225,58 @@ def test_window_build_lists_equal_per_slot_lists(cell_divisor):
     for a, b in zip(out[0][0], out[1][0]):
         assert np.array_equal(a, b)
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("kind,dims,N", [("berthier", 3, 6000), ("berthier", 2, 5000), ("lj", 3, 4000)])
+@pytest.mark.parametrize("cell_divisor", [1, 2, 3])
+def test_row_build_lists_equal_per_slot_lists(kind, dims, N, cell_divisor):
+    """The staged-row build (mode 4) writes the per-slot sweep's lists entry
+    for entry (same order), so forces agree bitwise; also after positions
+    moved (the slot order of a second rebuild) and with a rebuild requested
+    between steps."""
+    op, r, D, v, rc = H.make_system(kind, N, dims, seed=90 + cell_divisor + dims)
+    out = []
+    for mode in (4, 2):
+        ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=mode)
+        ctx.update_neighbor_lists()
+        ctx.do_time_steps(60)
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        out.append((ctx.get_neighbor_lists(), ctx.get_state(), ctx.rebuild_count()))
+        ctx.close()
+    for a, b in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a, b)
+    for f in ("r", "v", "F"):
+        assert np.array_equal(out[0][1][f], out[1][1][f]), f
+    assert out[0][2] == out[1][2]
+
+
+@pytest.mark.parametrize("N,L,cd,maxn", [(8000, 10.0, 1, 1000), (20000, 120.0, 4, 200)],
+                         ids=["dense-chunks", "dilute-wide-rows"])
+def test_row_build_chunked_and_wide_rows(N, L, cd, maxn):
+    """Staged-row build at its limits: a dense box (rho = 8, about 125
+    particles per cell at cell divisor 1) whose stencil rows hold more
+    candidates than one LDS chunk, and a dilute box at cell divisor 4 (207
+    cells per side) whose own rows span more raw cells than the LDS offset
+    table; the lists stay the oracle's and the per-slot sweep's."""
+    rng = np.random.default_rng(44 + cd)
+    dims = 3
+    D = synth.berthier_diameters(N, rng)
+    r = np.ascontiguousarray(rng.uniform(0.0, L, (N, dims)))
+    op = oracle.make_params(dims=dims, N=N, box_size=L, force_cutoff=1.25, skin=0.3, D=D,
+                            potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN, maxneighbours=maxn)
+    got = []
+    for mode in (4, 2):
+        ctx = make_ctx(op, 1.25, r, np.zeros_like(r), D, cell_divisor=cd, build_mode=mode)
+        ctx.update_neighbor_lists()
+        got.append(ctx.get_neighbor_lists())
+        ctx.close()
+    nf, full, nh, half = got[0]
+    lst = oracle.build_lists(op, r, D, cells=True)
+    assert np.array_equal(nf, lst.nfull)
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
+    for a, b in zip(got[0], got[1]):
+        assert np.array_equal(a, b)
 
 
 def test_window_overflow_falls_back_to_global_sweep():
@@ -304,8 +356,8 @@ def test_overflow_raises():
         ctx.update_neighbor_lists()
 
 
-@pytest.mark.parametrize("cell_divisor", [1, 2])
-def test_overflow_threshold_exact(cell_divisor):
+@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 3), (2, 3), (2, 4)])
+def test_overflow_threshold_exact(cell_divisor, build_mode):
     """maxneighbours equal to the longest full row builds; one less is an
     overflow (the windowed build counts full rows exactly whenever its
     candidate bound exceeds the capacity)."""
@@ -315,7 +367,7 @@ def test_overflow_threshold_exact(cell_divisor):
         op2 = oracle.make_params(dims=3, N=4000, box_size=op.box_size, force_cutoff=rc, skin=op.skin_distance,
                                  D=D, potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN,
                                  kBT=0.15, gamma=1.0, dt=0.005, maxneighbours=cap)
-        ctx = make_ctx(op2, rc, r, v, D, cell_divisor=cell_divisor, build_mode=3)
+        ctx = make_ctx(op2, rc, r, v, D, cell_divisor=cell_divisor, build_mode=build_mode)
         if raises:
             with pytest.raises(nat.NeighborOverflowError):
                 ctx.update_neighbor_lists()
