@@ -369,7 +369,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     c->geom.max_chunks = (h.ncell + SC_CHUNK - 1) / SC_CHUNK;
     c->geom.nruns = h.build_kind == BUILD_CELLS ? h.nruns : 0;
     c->geom.nwin = h.build_kind == BUILD_WIN ? h.nruns : 0;
-    c->geom.any_tile_build = h.build_kind == BUILD_SLOT ? 1 : 0;
+    c->geom.any_tile_build = h.build_kind == BUILD_SLOT ? (h.bf_on ? 2 : 1) : 0;
     c->geom.any_rows_build = h.build_kind == BUILD_ROWS ? 1 << h.dims : 0;
     c->geom.flag = reinterpret_cast<const int *>(reinterpret_cast<const char *>(h.dyn) + offsetof(RepDyn, rebuild));
     rc = SC_OK;
@@ -413,10 +413,11 @@ int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, c
         // largest diameter (swap moves keep the set of diameters), and the
         // mixed form's fixed-point diameters need a scale that holds it
         // (rescaled before the upload, which rewrites every diameter)
-        float dm = c->h.dmax;
+        float dm = c->h.dmax, dn = c->h.dmin;
         double dmx = 0.0;
         for (int64_t i = 0; i < c->h.n; i++) {
             dm = std::max(dm, (float)D[i]);
+            dn = std::min(dn, (float)D[i]);
             dmx = std::max(dmx, D[i]);
         }
         if (c->h.spos && dmx * c->h.dqinv >= 4294967295.0) {
@@ -424,8 +425,13 @@ int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, c
             if ((rc = push_rep(c))) return rc;
         }
         if ((rc = upload_field(c, D, FLD_D))) return rc;
-        if (dm > c->h.dmax) {
+        if (dm > c->h.dmax || dn < c->h.dmin) {
             c->h.dmax = dm;
+            c->h.dmin = dn;
+            // the prefilter's half-list margin follows the diameter range
+            // (beyond zeta (max D - min D) = 1/2 the FP64 sweep takes over)
+            sc::build_prefilter_setup(c->h);
+            if (c->h.build_kind == BUILD_SLOT) c->geom.any_tile_build = c->h.bf_on ? 2 : 1;
             // beyond zeta * max(D) = 2 the windowed build's FP32 margin no
             // longer covers sigma_ij: continue with the exact per-slot sweep
             if (c->h.build_kind == BUILD_WIN && c->h.pot == SC_POT_BERTHIER && c->h.zeta * (double)dm > 2.0) {

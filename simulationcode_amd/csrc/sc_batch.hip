@@ -117,7 +117,7 @@ int sync_tables(sc_batch *b)
         for (int t = 0; t < sl.h.ntiles; t++) tiles.push_back(Tile{r, t * SC_TB, t, 0});
         reps.push_back(sl.h);
         b->geom.max_chunks = std::max(b->geom.max_chunks, (sl.h.ncell + SC_CHUNK - 1) / SC_CHUNK);
-        if (sl.h.build_kind == BUILD_SLOT) b->geom.any_tile_build = 1;
+        if (sl.h.build_kind == BUILD_SLOT) b->geom.any_tile_build |= sl.h.bf_on ? 2 : 1;
         if (sl.h.build_kind == BUILD_ROWS) b->geom.any_rows_build |= 1 << sl.h.dims;
         if (sl.h.sys == SC_SYS_BROWNIAN)
             b->has_bd = true;

@@ -90,6 +90,18 @@ struct Rep {
                          // certainly > R_v^2; in between the FP64 test decides
     float hmarg;         // same margin for the half-list radius
     float dmax;          // largest diameter held so far (bounds every pair's half-list radius)
+    float dmin;          // smallest diameter held so far
+    // per-slot build with the fixed-point prefilter (bf_on): each candidate is
+    // first tested on its 16-byte rebuild-time record bpos (fixed-point
+    // coordinates, FP32 diameter) in FP32; d2 in fixed-point units^2 below
+    // bf_lo is certainly <= R_v^2, above bf_hi certainly > R_v^2; the
+    // half-list radius^2 rh2 (FP32, fixed-point units) is certainly passed
+    // below rh2 * bf_hlo - bf_m and certainly failed above rh2 * bf_hhi + bf_m
+    // (LJ: constant bounds bf_rh2lo / bf_rh2hi); in between the FP64 test on
+    // pos_alt decides
+    int bf_on;
+    float bf_lo, bf_hi, bf_hlo, bf_hhi, bf_m, bf_rh2lo, bf_rh2hi;
+    float bf_S, bf_rcut, bf_skin, bf_zeta;  // FP32 2^32 / L, sqrt(force_cutoff2), skin, zeta
     double wcell;        // cell side L / nc
     double L, invL, cinv;          // box, 1/box, cells per unit length
     double fc2, rcut, skin, Rv2, thr;
@@ -112,6 +124,8 @@ struct Rep {
     // ---- device buffers
     double4 *pos, *pos_alt;
     uint4 *spos;         // mixed precision only (else NULL): {ux, uy, uz, w} per slot, kept with pos
+    uint4 *bpos;         // per-slot build prefilter (else NULL): {ux, uy, uz, FP32 bits of D} of pos_alt,
+                         // written with the permutation at every rebuild
     double *v, *v_alt, *F, *disp;
     int *orig, *orig_alt;
     int *cell_of, *rank, *cell_cnt, *cell_start, *chunk_off, *key;
@@ -147,7 +161,8 @@ struct RebuildGeom {
     int max_chunks = 0;      // max over replicas of ceil(ncell / SC_CHUNK)
     int nruns = 0;           // LDS cell-run build runs over all replicas (BUILD_CELLS)
     int nwin = 0;            // windowed-build runs over all replicas (BUILD_WIN)
-    int any_tile_build = 0;  // some replica uses the per-slot build (all-pairs / explicit)
+    int any_tile_build = 0;  // replicas use the per-slot build: bit 0 the FP64 sweep (k_build), bit 1 the
+                             // fixed-point prefilter (k_build_fx)
     int any_rows_build = 0;  // replicas use the staged-row build: bit 2 some 2D replica, bit 3 some 3D one
     int *plan = nullptr;     // device work list, plan_ints(nreps, ntiles, nwin) ints (k_rebuild_plan)
     const int *flag = nullptr;  // one replica without a plan: its RepDyn::rebuild on the device
@@ -214,6 +229,10 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
 // fixed-point diameter scale of the mixed form for diameters up to maxD, and
 // one diameter in it (the device's rint(D * dqinv), bit for bit)
 void set_diameter_scale(Rep &h, double maxD);
+// FP32 margins of the per-slot build's fixed-point prefilter from the current
+// diameter range; clears bf_on where they do not hold (zeta < 0 or
+// zeta * (max D - min D) > 1/2 for the polydisperse potential)
+void build_prefilter_setup(Rep &h);
 unsigned diameter_fixed(double D, double dqinv);
 void free_all(std::vector<void *> &allocs);
 }  // namespace sc

@@ -31,6 +31,34 @@ __device__ __forceinline__ SC_AS1 T *gp(T *p)
     return (SC_AS1 T *)p;
 }
 
+#ifndef SC_BUILD_UNROLL
+#define SC_BUILD_UNROLL 4  // candidates loaded together per lane in the prefiltered sweep
+#endif
+
+// Launch tables (tiles, replicas, rebuild plan) are written only between
+// launches, so kernels may read them as constant memory: a read at a
+// wavefront-uniform index then becomes a scalar load into SGPRs, also inside
+// loops that store to global memory, where the compiler could not otherwise
+// rule out that a store changed the table.
+#define SC_AS4 __attribute__((address_space(4)))
+template <typename T>
+__device__ __forceinline__ const SC_AS4 T *cp(const T *p)
+{
+    return (const SC_AS4 T *)p;
+}
+// table entry i as a value (word by word: the unused words are never loaded)
+template <typename T>
+__device__ __forceinline__ T ldc(const T *p, int i)
+{
+    static_assert(sizeof(T) % 4 == 0, "4-byte words");
+    T v;
+    const SC_AS4 unsigned *src = (const SC_AS4 unsigned *)(p + i);
+    unsigned *dst = reinterpret_cast<unsigned *>(&v);
+#pragma unroll
+    for (int w = 0; w < (int)(sizeof(T) / 4); w++) dst[w] = src[w];
+    return v;
+}
+
 // double4 loads / stores through global pointers (the vector type's own
 // copy operations take generic references)
 typedef double sc_dbl4v __attribute__((ext_vector_type(4)));
@@ -884,7 +912,7 @@ __global__ __launch_bounds__(SC_TB) void k_rebuild_plan(const Rep *reps, int nre
 struct RebuildWork {
     const int *tl;
     int count;
-    __device__ int tile(int w) const { return tl ? tl[w] : w; }
+    __device__ int tile(int w) const { return tl ? cp(tl)[w] : w; }
 };
 
 // flag: the single replica's rebuild flag, passed directly so that a step
@@ -938,8 +966,8 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
 {
     const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[wk.tile(it.cur)];
-        const Rep R = reps[t.rep];
+        const Tile t = ldc(tiles, wk.tile(it.cur));
+        const Rep R = ldc(reps, t.rep);
         const int i = t.first + threadIdx.x;
         const bool valid = i < R.n;
         int c = -1;
@@ -1076,8 +1104,8 @@ __global__ __launch_bounds__(SC_TB) void k_scatter(const Tile *tiles, const Rep 
 {
     const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[wk.tile(it.cur)];
-        const Rep R = reps[t.rep];
+        const Tile t = ldc(tiles, wk.tile(it.cur));
+        const Rep R = ldc(reps, t.rep);
         if (fused && t.local == 0) {
             const int nch = (R.ncell + SC_CHUNK - 1) / SC_CHUNK;
             for (int q = threadIdx.x; q < nch * SC_CHUNK_SPLIT; q += SC_TB) gp(R.chunk_cnt)[q] = 0;
@@ -1099,8 +1127,8 @@ __global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const
 {
     const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[wk.tile(it.cur)];
-        const Rep R = reps[t.rep];
+        const Tile t = ldc(tiles, wk.tile(it.cur));
+        const Rep R = ldc(reps, t.rep);
         const int x = t.first + threadIdx.x, n = R.n;
         if (x >= n) continue;
         const int c = gp(R.cell_of)[x];
@@ -1109,7 +1137,12 @@ __global__ __launch_bounds__(SC_TB) void k_rank_permute(const Tile *tiles, const
         int rank = 0;
         for (int a = s0; a < e0; a++) rank += gp(R.key)[a] < oid ? 1 : 0;
         const int k = s0 + rank;
-        st4(gp(R.pos_alt) + (k), ld4(gp(R.pos) + (x)));
+        const double4 q = ld4(gp(R.pos) + (x));
+        st4(gp(R.pos_alt) + (k), q);
+        if (R.bpos)  // the per-slot build's prefilter record of pos_alt[k]
+            *(SC_AS1 sc_u32x4v *)(R.bpos + k) =
+                sc_u32x4v{fixed_coord(q.x, R.uq), fixed_coord(q.y, R.uq), R.dims == 3 ? fixed_coord(q.z, R.uq) : 0u,
+                          __float_as_uint((float)q.w)};
         for (int d = 0; d < R.dims; d++) gp(R.v_alt)[d * n + k] = gp(R.v)[d * n + x];
         if (gp(R.dyn)->rebuild & REBUILD_KEEP_F)
             for (int d = 0; d < R.dims; d++) gp(R.stage)[d * n + k] = gp(R.F)[d * n + x];
@@ -1304,14 +1337,72 @@ __global__ __launch_bounds__(SC_TB) void k_build_cells(const Rep *reps, int nrep
     __shared__ BuildLds S;
     for (XcdRange it = xcd_range(nruns_total); it.cur < it.end; it.cur += it.step) {
         int r = 0;
-        while (r + 1 < nreps && reps[r + 1].run0 <= it.cur) r++;
-        const Rep R = reps[r];
+        while (r + 1 < nreps && ldc(reps, r + 1).run0 <= it.cur) r++;
+        const Rep R = ldc(reps, r);
         if (R.build_kind != BUILD_CELLS || !need_rebuild(R)) continue;
         if (R.dims == 3)
             build_run<3, 8>(R, it.cur - R.run0, S);
         else
             build_run<2, 4>(R, it.cur - R.run0, S);
     }
+}
+
+// The FP64 predicates of slot k's pair with slot j (reference minimum image),
+// for candidates the prefilter cannot decide: bit 0 full list, bit 1 half list.
+template <int DIMS>
+__device__ __forceinline__ int exact_pair(const Rep &R, int k, int j)
+{
+    const double4 p = ld4(gp(R.pos_alt) + (k)), q = ld4(gp(R.pos_alt) + (j));
+    double dx, dy, dz;
+    const double d2 = sqdist<DIMS>(R, p, q, dx, dy, dz);
+    if (!(d2 <= R.Rv2)) return 0;
+    const double md = (R.pot == 2) ? (p.w + q.w) * 0.5 * (1.0 - R.zeta * fabs(p.w - q.w)) : R.lj_md;
+    const double rh = R.rcut * md + R.skin;
+    return d2 < rh * rh ? 3 : 1;
+}
+
+// Fixed-point prefilter (R.bf_on): the pair's separation from the wrapping
+// differences of the rebuild-time fixed-point coordinates (the minimum image
+// without a shift), squared and summed in FP32, and the half-list radius in
+// FP32 from the FP32 diameters; a result within the margins of a threshold
+// is decided by exact_pair.  The outcome (bit 0 full list, bit 1 half list)
+// is exactly the FP64 predicates'.
+template <int DIMS>
+__device__ __forceinline__ int prefilter_q(const Rep &R, int k, const uint4 &pk, float sk, int j, const uint4 &q)
+{
+    const float fx = (float)(int)(pk.x - q.x);
+    const float fy = (float)(int)(pk.y - q.y);
+    float d2 = fx * fx + fy * fy;
+    if (DIMS == 3) {
+        const float fz = (float)(int)(pk.z - q.z);
+        d2 = d2 + fz * fz;
+    }
+    if (d2 > R.bf_hi) return 0;
+    int res = 1;
+    bool exact = d2 >= R.bf_lo;
+    if (!exact) {
+        float lo = R.bf_rh2lo, hi = R.bf_rh2hi;
+        if (R.pot == 2) {
+            const float sj = __uint_as_float(q.w);
+            const float md = (sk + sj) * 0.5f * (1.0f - R.bf_zeta * fabsf(sk - sj));
+            const float rh = (R.bf_rcut * md + R.bf_skin) * R.bf_S;
+            const float rh2 = rh * rh;
+            lo = rh2 * R.bf_hlo - R.bf_m;
+            hi = rh2 * R.bf_hhi + R.bf_m;
+        }
+        if (d2 < lo)
+            res = 3;
+        else
+            exact = d2 <= hi;
+    }
+    if (exact) res = exact_pair<DIMS>(R, k, j);
+    return res;
+}
+
+template <int DIMS>
+__device__ __forceinline__ int prefilter_pair(const Rep &R, int k, const uint4 &pk, float sk, int j)
+{
+    return prefilter_q<DIMS>(R, k, pk, sk, j, ldq(R.bpos + j));
 }
 
 // Cell-image displacement.  With a stencil of +-s cells (cell side w > R_v/s,
@@ -1442,6 +1533,91 @@ __device__ void build_slot(const Rep &R, int k)
     build_finish_slot<DIMS>(R, k, p, nfull, cnt);
 }
 
+// The stencil rows of for_stencil_rows as slot ranges only (the prefilter
+// needs no image shift), trimmed in FP32 with 1e-4 cell units of slack toward
+// inclusion, far above the FP32 rounding of these quantities: a superset of
+// the FP64 trim's candidates, all within R_v among the extra ones dropped by
+// the predicates, in the same order.  (fx, fy, fz): the slot's fractional
+// position in its cell cc.
+__device__ inline float cell_gap32(int o, float f)
+{
+    const float g = o > 0 ? (float)o - f : (o < 0 ? f - (float)(o + 1) : 0.0f);
+    return g > 1e-4f ? g - 1e-4f : 0.0f;
+}
+
+template <int DIMS, class Seg>
+__device__ __forceinline__ void for_stencil_ranges(const Rep &R, int c0, int c1, int c2, float fx, float fy, float fz, Seg &&seg)
+{
+    const int s = R.srange, nc = R.nc;
+    const int zr = DIMS == 3 ? s : 0;
+    const float rc2 = (float)(R.Rv2 * R.cinv * R.cinv) * (1.0f + 1e-4f) + 1e-4f;
+    for (int oz = -zr; oz <= zr; oz++) {
+        const int cz = DIMS == 3 ? wrapc(c2 + oz, nc) : 0;
+        const float gz = DIMS == 3 ? cell_gap32(oz, fz) : 0.0f;
+        for (int oy = -s; oy <= s; oy++) {
+            const float gy = cell_gap32(oy, fy);
+            const float g2 = gy * gy + gz * gz;
+            if (g2 > rc2) continue;
+            const float rx = sqrtf(rc2 - g2) + 1e-4f;
+            const int lo = max(-s, (int)floorf(fx - rx)), hi = min(s, (int)floorf(fx + rx));
+            if (lo > hi) continue;
+            const int row = nc * (wrapc(c1 + oy, nc) + nc * cz);
+            const int x0 = c0 + lo, x1 = c0 + hi;
+            if (x0 < 0) seg(cstart(R, row + x0 + nc), cstart(R, row + min(x1, -1) + nc + 1));
+            const int b0 = max(x0, 0), b1 = min(x1, nc - 1);
+            if (b0 <= b1) seg(cstart(R, row + b0), cstart(R, row + b1 + 1));
+            if (x1 >= nc) seg(cstart(R, row + max(x0, nc) - nc), cstart(R, row + x1 - nc + 1));
+        }
+    }
+}
+
+// Cell and fractional in-cell position of slot k (the stencil's origin).
+template <int DIMS>
+__device__ inline void slot_cell(const Rep &R, int k, int &c0, int &c1, int &c2, float &fx, float &fy, float &fz)
+{
+    const double4 p = ld4(gp(R.pos_alt) + (k));
+    c0 = cell_coord(p.x, R);
+    c1 = cell_coord(p.y, R);
+    c2 = DIMS == 3 ? cell_coord(p.z, R) : 0;
+    fx = (float)(p.x * R.cinv - (double)c0);
+    fy = (float)(p.y * R.cinv - (double)c1);
+    fz = DIMS == 3 ? (float)(p.z * R.cinv - (double)c2) : 0.0f;
+}
+
+// One thread per slot with the fixed-point prefilter (R.bf_on, stencil >= 1):
+// the same candidates in the same order, so the same list.
+template <int DIMS>
+__device__ void build_slot_fx(const Rep &R, int k)
+{
+    int c0, c1, c2;
+    float fx, fy, fz;
+    slot_cell<DIMS>(R, k, c0, c1, c2, fx, fy, fz);
+    const uint4 pk = ldq(R.bpos + k);
+    const float sk = __uint_as_float(pk.w);
+    int nfull = 0, cnt = 0;
+    // SC_BUILD_UNROLL candidate records are loaded together (indices clamped
+    // to the range, so every load is in bounds) and then tested in order
+    for_stencil_ranges<DIMS>(R, c0, c1, c2, fx, fy, fz, [&](int b, int e) __attribute__((always_inline)) {
+        for (int j0 = b; j0 < e; j0 += SC_BUILD_UNROLL) {
+            uint4 q[SC_BUILD_UNROLL];
+#pragma unroll
+            for (int u = 0; u < SC_BUILD_UNROLL; u++) q[u] = ldq(R.bpos + min(j0 + u, e - 1));
+#pragma unroll
+            for (int u = 0; u < SC_BUILD_UNROLL; u++) {
+                const int j = j0 + u;
+                if (j >= e || j == k) continue;
+                const int res = prefilter_q<DIMS>(R, k, pk, sk, j, q[u]);
+                nfull += res & 1;
+                if (res & 2) {
+                    if (cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = j;
+                    cnt++;
+                }
+            }
+        }
+    });
+    build_finish_slot<DIMS>(R, k, ld4(gp(R.pos_alt) + (k)), nfull, cnt);
+}
+
 // LPP lanes per slot: lane l of the group takes candidates b+l, b+l+LPP, ...
 // of a range; accepted entries are placed with a ballot in candidate order,
 // so the list is identical to the one-thread sweep's, with LPP candidate
@@ -1488,6 +1664,26 @@ __device__ inline void build_segment(const Rep &R, int k, const double4 p, int b
     }
 }
 
+// The same with the fixed-point prefilter (R.bf_on).
+template <int DIMS, int LPP>
+__device__ inline void build_segment_fx(const Rep &R, int k, const uint4 &pk, float sk, int b, int e, int l,
+                                        unsigned long long gmask, unsigned long long below, int &nfull, int &cnt)
+{
+    for (int a0 = b; a0 < e; a0 += LPP) {
+        const int a = a0 + l;
+        const int res = (a < e && a != k) ? prefilter_pair<DIMS>(R, k, pk, sk, a) : 0;
+        const bool inF = (res & 1) != 0, inH = (res & 2) != 0;
+        const unsigned long long mF = __ballot(inF) & gmask;
+        const unsigned long long mH = __ballot(inH) & gmask;
+        if (inH) {
+            const int slot = cnt + __popcll(mH & below);
+            if (slot < R.cap) gp(R.nbr)[(size_t)slot * R.n + k] = a;
+        }
+        nfull += __popcll(mF);
+        cnt += __popcll(mH);
+    }
+}
+
 template <int DIMS, int LPP>
 __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long gmask, unsigned long long below)
 {
@@ -1501,6 +1697,21 @@ __device__ void build_slot_lanes(const Rep &R, int k, int l, unsigned long long 
         });
     }
     if (l == 0) build_finish_slot<DIMS>(R, k, p, nfull, cnt);
+}
+
+template <int DIMS, int LPP>
+__device__ void build_slot_lanes_fx(const Rep &R, int k, int l, unsigned long long gmask, unsigned long long below)
+{
+    int c0, c1, c2;
+    float fx, fy, fz;
+    slot_cell<DIMS>(R, k, c0, c1, c2, fx, fy, fz);
+    const uint4 pk = ldq(R.bpos + k);
+    const float sk = __uint_as_float(pk.w);
+    int nfull = 0, cnt = 0;
+    for_stencil_ranges<DIMS>(R, c0, c1, c2, fx, fy, fz, [&](int b, int e) __attribute__((always_inline)) {
+        build_segment_fx<DIMS, LPP>(R, k, pk, sk, b, e, l, gmask, below, nfull, cnt);
+    });
+    if (l == 0) build_finish_slot<DIMS>(R, k, ld4(gp(R.pos_alt) + (k)), nfull, cnt);
 }
 
 // R5 (per-slot form) over the planned tiles.  Launch shape: BS threads per
@@ -1520,9 +1731,9 @@ __global__ __launch_bounds__(BS, LPP == 1 ? 5 : 1) void k_build(const Tile *tile
     const unsigned long long gmask = LPP == WAVE ? ~0ull : ((1ull << LPP) - 1ull) << (lane / LPP * LPP);
     const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
     for (XcdRange it = xcd_range(wk.count * IPT); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[wk.tile(it.cur / IPT)];
-        const Rep R = reps[t.rep];
-        if (R.build_kind != BUILD_SLOT) continue;
+        const Tile t = ldc(tiles, wk.tile(it.cur / IPT));
+        const Rep R = ldc(reps, t.rep);
+        if (R.build_kind != BUILD_SLOT || R.bf_on) continue;
         const int k = t.first + (it.cur % IPT) * SPI + (int)threadIdx.x / LPP;
         if (k >= R.n) continue;
         if (LPP == 1) {
@@ -1535,6 +1746,39 @@ __global__ __launch_bounds__(BS, LPP == 1 ? 5 : 1) void k_build(const Tile *tile
                 build_slot_lanes<3, LPP>(R, k, l, gmask, below);
             else
                 build_slot_lanes<2, LPP>(R, k, l, gmask, below);
+        }
+    }
+}
+
+// The per-slot form with the fixed-point prefilter, for the replicas with
+// R.bf_on (k_build skips them); same launch shapes and work items.
+template <int BS, int LPP>
+__global__ __launch_bounds__(BS) void k_build_fx(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
+                                                 int nreps, int ntiles)
+{
+    constexpr int SPI = BS / LPP;
+    constexpr int IPT = SC_TB / SPI;
+    static_assert(SPI * IPT == SC_TB && LPP <= WAVE, "launch shape");
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
+    const int lane = threadIdx.x % WAVE, l = threadIdx.x % LPP;
+    const unsigned long long gmask = LPP == WAVE ? ~0ull : ((1ull << LPP) - 1ull) << (lane / LPP * LPP);
+    const unsigned long long below = ((1ull << lane) - 1ull) & gmask;
+    for (XcdRange it = xcd_range(wk.count * IPT); it.cur < it.end; it.cur += it.step) {
+        const Tile t = ldc(tiles, wk.tile(it.cur / IPT));
+        const Rep R = ldc(reps, t.rep);
+        if (R.build_kind != BUILD_SLOT || !R.bf_on) continue;
+        const int k = t.first + (it.cur % IPT) * SPI + (int)threadIdx.x / LPP;
+        if (k >= R.n) continue;
+        if (LPP == 1) {
+            if (R.dims == 3)
+                build_slot_fx<3>(R, k);
+            else
+                build_slot_fx<2>(R, k);
+        } else {
+            if (R.dims == 3)
+                build_slot_lanes_fx<3, LPP>(R, k, l, gmask, below);
+            else
+                build_slot_lanes_fx<2, LPP>(R, k, l, gmask, below);
         }
     }
 }
@@ -1696,8 +1940,8 @@ __global__ __launch_bounds__(WAVE) void k_build_rows(const Tile *tiles, const Re
     constexpr int GPT = SC_TB / WAVE;  // 64-slot groups per tile
     const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
     for (XcdRange it = xcd_range(wk.count * GPT); it.cur < it.end; it.cur += it.step) {
-        const Tile t = tiles[wk.tile(it.cur / GPT)];
-        const Rep R = reps[t.rep];
+        const Tile t = ldc(tiles, wk.tile(it.cur / GPT));
+        const Rep R = ldc(reps, t.rep);
         const int k0 = t.first + (it.cur % GPT) * WAVE;
         // uniform over the block (one wave)
         if (R.build_kind != BUILD_ROWS || R.dims != DIMS || k0 >= R.n) continue;
@@ -2101,9 +2345,9 @@ __global__ __launch_bounds__(SC_TB, 3) void k_build_win(const Rep *reps, const i
     const int count = plan ? plan[2]
                            : ((*gp(flag) != 0 && reps[0].build_kind == BUILD_WIN) ? reps[0].nruns : 0);
     for (XcdRange it = xcd_range(count); it.cur < it.end; it.cur += it.step) {
-        const int r = rl ? rl[2 * it.cur] : 0;
-        const int run = rl ? rl[2 * it.cur + 1] : it.cur;
-        const Rep R = reps[r];
+        const int r = rl ? cp(rl)[2 * it.cur] : 0;
+        const int run = rl ? cp(rl)[2 * it.cur + 1] : it.cur;
+        const Rep R = ldc(reps, r);
         if (R.dims == 3)
             win_run<3>(R, run, S);
         else
@@ -3019,12 +3263,14 @@ bool build_shape_valid(int bs, int lpp)
 
 template <int BS, int LPP>
 static void launch_build_shape(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const int *plan,
-                               const int *flag, hipStream_t s)
+                               const int *flag, hipStream_t s, int which)
 {
     const int items = ntiles * (SC_TB * LPP / BS);
     const int cap = SC_GRID_CAP * (SC_TB / BS);
-    hipLaunchKernelGGL((k_build<BS, LPP>), dim3(items < cap ? items : cap), dim3(BS), 0, s, tiles, reps, plan, flag,
-                       nreps, ntiles);
+    const dim3 grid(items < cap ? items : cap);
+    if (which & 1) hipLaunchKernelGGL((k_build<BS, LPP>), grid, dim3(BS), 0, s, tiles, reps, plan, flag, nreps, ntiles);
+    if (which & 2)
+        hipLaunchKernelGGL((k_build_fx<BS, LPP>), grid, dim3(BS), 0, s, tiles, reps, plan, flag, nreps, ntiles);
 }
 
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,
@@ -3061,20 +3307,20 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
             hipLaunchKernelGGL(k_build_rows<2>, grid, dim3(64), 0, s, tiles, reps, plan, flag, nreps, ntiles);
     }
     if (g.any_tile_build) {
-        const int key = g.build_bs * 64 + g.build_lpp;
+        const int key = g.build_bs * 64 + g.build_lpp, w = g.any_tile_build;
         switch (key) {
-        case 64 * 64 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 64 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 64 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 64 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 64 + 16: launch_build_shape<64, 16>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 64 * 64 + 32: launch_build_shape<64, 32>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 64 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 64 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 64 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 64 + 8: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        case 256 * 64 + 16: launch_build_shape<256, 16>(tiles, ntiles, reps, nreps, plan, flag, s); break;
-        default: launch_build_shape<256, 32>(tiles, ntiles, reps, nreps, plan, flag, s); break;
+        case 64 * 64 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 64 * 64 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 64 * 64 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 64 * 64 + 8: launch_build_shape<64, 8>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 64 * 64 + 16: launch_build_shape<64, 16>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 64 * 64 + 32: launch_build_shape<64, 32>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 256 * 64 + 1: launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 256 * 64 + 2: launch_build_shape<256, 2>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 256 * 64 + 4: launch_build_shape<256, 4>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 256 * 64 + 8: launch_build_shape<256, 8>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        case 256 * 64 + 16: launch_build_shape<256, 16>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
+        default: launch_build_shape<256, 32>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
         }
     }
     if (bookkeep) fin(reps, nreps, FIN_REBUILD_DONE, SYS_ALL, 0.0, s);

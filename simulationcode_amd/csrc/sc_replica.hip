@@ -57,6 +57,7 @@ int64_t replica_bytes(const sc_params *p, const double *D)
     b += (2 * ncell + 1) * 4 + 4 * (1 + SC_CHUNK_SPLIT) * ((ncell + SC_CHUNK - 1) / SC_CHUNK);
     b += n * 4 + n * p->maxneighbours * 4;  // nn, nbr
     if (p->precision == SC_PREC_MIXED) b += n * 16;  // spos
+    if (p->build_mode == 0 && srange >= 1) b += n * 16;  // bpos
     b += ntiles * 8 * 8 + n * 3 * 8 + 4096;  // partials, staging, dyn
     return b;
 }
@@ -74,6 +75,44 @@ void set_diameter_scale(Rep &h, double maxD)
 }
 
 unsigned diameter_fixed(double D, double dqinv) { return (unsigned)(long long)std::rint(D * dqinv); }
+
+// Margins of the per-slot build's fixed-point prefilter.  A coordinate's
+// u = rint(x S) (S = 2^32 / L, x S rounded in FP64) lies within 0.51 of x S,
+// so the wrapping difference of two lies within 1.02 of the minimum-image
+// separation times S; its FP32 value adds 2^-24 relative, and the squares and
+// the sum in FP32 three roundings of 2^-24.  For a separation d, D2 = d^2 S^2:
+//   |d2_32 - D2| <= 6 2^-24 D2 + 2.1 sqrt(dims) sqrt(D2) + 1.1 dims =: M(D2),
+// increasing in D2, and the FP64 d2 of the same positions differs from D2 by
+// far less.  Four times M(R_v^2 S^2) is taken on both sides of R_v^2 S^2.
+// The half-list radius rcut sigma_ij + skin evaluated in FP32 from FP32
+// diameters, times S and squared, is off by at most (23 + 16 zeta max D)
+// 2^-24 relative while 1 - zeta |D_i - D_j| >= 1/2 (the subtraction then
+// cannot amplify the error more than twofold); eight times that is the
+// relative margin, plus the absolute 4 M(R_v^2 S^2).  The LJ radius is one
+// FP64 constant, given 2^-20 relative slack and the same absolute margin.
+void build_prefilter_setup(Rep &h)
+{
+    const double S = h.uq, Rv = std::sqrt(h.Rv2), u = std::ldexp(1.0, -24);
+    const double RvS2 = h.Rv2 * S * S;
+    const double m4 = 4.0 * (6.0 * u * RvS2 + 2.1 * std::sqrt((double)h.dims) * Rv * S + 1.1 * h.dims);
+    h.bf_lo = (float)(RvS2 - m4);
+    h.bf_hi = (float)(RvS2 + m4);
+    h.bf_m = (float)m4;
+    h.bf_S = (float)S;
+    h.bf_rcut = (float)h.rcut;
+    h.bf_skin = (float)h.skin;
+    h.bf_zeta = (float)h.zeta;
+    const double dmax = h.dmax, dmin = h.dmin;
+    const double eh = 8.0 * (23.0 + 16.0 * std::fabs(h.zeta) * dmax) * u;
+    h.bf_hlo = (float)(1.0 - eh);
+    h.bf_hhi = (float)(1.0 + eh);
+    const double rh = h.rcut * h.lj_md + h.skin, rh2 = rh * rh * S * S;
+    h.bf_rh2lo = (float)(rh2 * (1.0 - std::ldexp(1.0, -20)) - m4);
+    h.bf_rh2hi = (float)(rh2 * (1.0 + std::ldexp(1.0, -20)) + m4);
+    bool ok = h.bpos != nullptr && h.srange >= 1 && h.build_kind == BUILD_SLOT;
+    if (h.pot == SC_POT_BERTHIER) ok = ok && h.zeta >= 0.0 && h.zeta * (dmax - dmin) <= 0.5;
+    h.bf_on = ok ? 1 : 0;
+}
 
 static int alloc_into(std::vector<void *> &allocs, void **ptr, size_t bytes, std::string &err)
 {
@@ -149,8 +188,11 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
 {
     std::memset(&h, 0, sizeof(Rep));
     const int n = (int)p->N;
-    double maxD = D[0];
-    for (int i = 1; i < n; i++) maxD = std::max(maxD, D[i]);
+    double maxD = D[0], minD = D[0];
+    for (int i = 1; i < n; i++) {
+        maxD = std::max(maxD, D[i]);
+        minD = std::min(minD, D[i]);
+    }
     // initialize_neighbor_struct, src/DataStructs.jl:351-352
     const double Rv = p->skin_distance + p->force_cutoff * maxD;
     if (!(p->box_size > 2.0 * Rv)) {
@@ -208,6 +250,7 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     // per-slot sweep
     h.wcell = p->box_size / (double)nc;
     h.dmax = (float)maxD;
+    h.dmin = (float)minD;
     // the windowed build's FP32 half-list radius stays within its margin while
     // 0 <= zeta * max(D) <= 2 (sigma_ij error <= dmax (6 zeta dmax + 4) 2^-24,
     // threshold error <= R_v^2 (12 zeta dmax + 15) 2^-24 < 4 R_v^2 2^-20);
@@ -255,7 +298,12 @@ int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *
     rc |= A(allocs, &h.dyn, 1, err);
     h.spos = nullptr;
     if (h.prec == SC_PREC_MIXED) rc |= A(allocs, &h.spos, n, err);
+    // automatic build mode: the per-slot sweep with the fixed-point prefilter
+    // (build mode 2 keeps the FP64-only sweep)
+    h.bpos = nullptr;
+    if (h.build_kind == BUILD_SLOT && srange >= 1 && p->build_mode == 0) rc |= A(allocs, &h.bpos, n, err);
     if (rc != SC_OK) return SC_ERR_OOM;
+    build_prefilter_setup(h);
 
     std::vector<int> ident(n);
     for (int i = 0; i < n; i++) ident[i] = i;

@@ -251,6 +251,64 @@ def test_row_build_lists_equal_per_slot_lists(kind, dims, N, cell_divisor):
     assert out[0][2] == out[1][2]
 
 
+@pytest.mark.parametrize("kind,dims,N", [("berthier", 3, 6000), ("berthier", 2, 5000), ("lj", 3, 4000),
+                                         ("lj", 2, 3000)])
+@pytest.mark.parametrize("cell_divisor", [1, 2, 3])
+def test_prefilter_build_equals_fp64_sweep(kind, dims, N, cell_divisor):
+    """The automatic per-slot build (mode 0) tests each candidate on its
+    fixed-point record in FP32 within proven error margins and decides the
+    pairs inside the margins with the FP64 predicates; its lists equal the
+    FP64-only sweep's (mode 2) entry for entry, so the trajectories agree
+    bitwise, also over the rebuilds of 60 steps and a rebuild requested
+    between steps."""
+    op, r, D, v, rc = H.make_system(kind, N, dims, seed=120 + cell_divisor + dims)
+    out = []
+    for mode in (0, 2):
+        ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=mode)
+        ctx.update_neighbor_lists()
+        first = ctx.get_neighbor_lists()
+        ctx.do_time_steps(60)
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        out.append((first, ctx.get_neighbor_lists(), ctx.get_state(), ctx.rebuild_count()))
+        ctx.close()
+    for k in (0, 1):
+        for a, b in zip(out[0][k], out[1][k]):
+            assert np.array_equal(a, b)
+    for f in ("r", "v", "F"):
+        assert np.array_equal(out[0][2][f], out[1][2][f]), f
+    assert out[0][3] == out[1][3] >= 2
+
+
+def test_prefilter_follows_the_diameter_range():
+    """Diameters set past zeta (max D - min D) = 1/2, where the prefilter's
+    FP32 half-list margin would not hold, return the automatic build to the
+    FP64 sweep; before and after, its lists are the mode-2 sweep's and the
+    oracle's."""
+    op, r, D, v, rc = H.make_system("berthier", 3000, 3, seed=5)
+    D2 = D.copy()
+    D2[:10], D2[10:20] = 0.1, 2.8
+    assert op.b_zeta * (D2.max() - D2.min()) > 0.5
+    out = []
+    for mode in (0, 2):
+        ctx = make_ctx(op, rc, r, v, D, cell_divisor=2, build_mode=mode)
+        ctx.update_neighbor_lists()
+        before = ctx.get_neighbor_lists()
+        ctx.set_state(D=D2)
+        ctx.update_neighbor_lists()
+        out.append((before, ctx.get_neighbor_lists()))
+        ctx.close()
+    for k in (0, 1):
+        for a, b in zip(out[0][k], out[1][k]):
+            assert np.array_equal(a, b)
+    op2 = oracle.make_params(dims=3, N=3000, box_size=op.box_size, force_cutoff=1.25, skin=0.3, D=D,
+                             potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN)
+    lst = oracle.build_lists(op2, r, D2, cells=False)
+    nf, full, nh, half = out[0][1]
+    assert np.array_equal(nf, lst.nfull)
+    assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs())
+
+
 @pytest.mark.parametrize("N,L,cd,maxn", [(8000, 10.0, 1, 1000), (20000, 120.0, 4, 200)],
                          ids=["dense-chunks", "dilute-wide-rows"])
 def test_row_build_chunked_and_wide_rows(N, L, cd, maxn):
