@@ -362,6 +362,61 @@ def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300
                          "bytes_per_particle_step_mean": round(byt / tot, 2)}}
 
 
+def run_batched_stated_dt(nat, specs, cd, shape, steps=300, call=25, relax=(300, 0.005, 0.02)):
+    """The batched workload at its stated dt = 0.001: one batch stepped in
+    calls of `call` steps; replicas whose state goes non-finite (or whose
+    lists overflow) are retired after the call, as the server does, and the
+    others go on.  Reports the failures, the rebuild rate and the throughput
+    of the steps made."""
+    b = nat.Batch(len(specs) + 1)
+    b.set_build_launch(*shape)
+    ids, Ns = [], {}
+    for s in specs:
+        p, D, r, v = W_replica(dict(s, dt=0.001), cd)
+        rid = b.add(p, D, r, v)
+        ids.append(rid)
+        Ns[rid] = s["N"]
+    b.minimize(*relax)
+    alive = set(ids)
+    failed, done_steps, rebuilds = {}, {}, {}
+    wall = 0.0
+    made = 0
+    while made < steps and alive:
+        k = min(call, steps - made)
+        t0 = time.perf_counter()
+        try:
+            b.step(k)
+        except nat.ScError:
+            pass
+        wall += time.perf_counter() - t0
+        made += k
+        for rid in sorted(alive):
+            code = b.replica_error(rid)
+            if code != nat.SC_OK:
+                st = b.stats(rid)
+                failed[rid] = {"code": int(code), "steps": int(st["steps"])}
+                rebuilds[rid] = int(st["rebuilds"])
+                done_steps[rid] = int(st["steps"])
+                b.remove(rid)
+                alive.discard(rid)
+    for rid in alive:
+        st = b.stats(rid)
+        done_steps[rid] = int(st["steps"])
+        rebuilds[rid] = int(st["rebuilds"])
+    b.close()
+    psteps = sum(Ns[r] * done_steps[r] for r in ids)
+    fsteps = sorted(f["steps"] for f in failed.values())
+    mean_steps = sum(done_steps.values()) / len(ids)
+    return {"dt": 0.001, "steps": steps, "call_steps": call, "replicas": len(ids),
+            "failed": len(failed), "failed_frac": round(len(failed) / len(ids), 4),
+            "failure_steps_p0_p50": [fsteps[0], fsteps[len(fsteps) // 2]] if fsteps else None,
+            "rebuilds_per_step": round(sum(rebuilds.values()) / mean_steps, 3) if mean_steps else None,
+            "rebuild_interval_steps_per_replica": round(mean_steps * len(ids) / max(sum(rebuilds.values()), 1), 2),
+            "value": psteps / wall if wall else None, "unit": "particle-steps/s",
+            "note": "steps made by every replica until the call after its failure; the stated dt with explicit "
+                    "Euler-Maruyama on the r^-12 core"}
+
+
 def W_replica(spec, cd):
     from simulationcode_amd import workloads as W
     return W.replica(**dict(spec, cell_divisor=cd))
@@ -374,11 +429,16 @@ def batched_summary(args, nat, steps=300, warmup=20):
     steps)."""
     from simulationcode_amd import workloads as W
     t0 = time.perf_counter()
-    res = run_batched(nat, W.batched_specs(args.replicas), 1, (256, 4), 2, False, warmup, steps)
+    specs = W.batched_specs(args.replicas)
+    res = run_batched(nat, specs, 1, (256, 4), 2, False, warmup, steps)
     res.update({"steps": steps, "warmup": warmup, "dt": 1e-4, "dynamics": "Brownian", "dtype": "mixed",
                 "cell_divisor": 1, "streams": 2, "wall_s": round(time.perf_counter() - t0, 1)})
     log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step, {res['value']:.3g} particle-steps/s, "
-        f"roofline {res['roofline']['frac']:.3f}")
+        f"roofline {res['roofline']['frac']:.3f}, {res['rebuilds_per_step']:.2f} rebuilds/step")
+    res["stated_dt"] = sd = run_batched_stated_dt(nat, specs, 1, (256, 4))
+    log(f"[bench] batched at dt=0.001: {sd['failed']}/{sd['replicas']} replicas non-finite within "
+        f"{sd['steps']} steps (first at step {sd['failure_steps_p0_p50'][0] if sd['failed'] else '-'}), "
+        f"{sd['rebuilds_per_step']} rebuilds/step, {sd['value']:.3g} particle-steps/s")
     return res
 
 

@@ -53,10 +53,13 @@ typedef struct sc_params {
     int32_t precision;            /* SC_PREC_* */
     int32_t rescale_temperature;  /* Newtonian.rescale_temperature */
     int32_t cell_divisor;         /* cells of side >= R_v / cell_divisor, stencil +-cell_divisor; 0 => 2 */
-    int32_t build_mode;           /* 0: automatic (the per-slot sweep); 1: LDS-staged cell runs (cell_divisor 1);
-                                     2: per-slot stencil sweep; 3: LDS windows with an FP32 prefilter
-                                     (cell_divisor 1-3); 4: LDS-staged stencil rows, one wavefront per
-                                     64 slots; every mode writes the same lists */
+    int32_t build_mode;           /* 0: automatic (the per-slot sweep testing candidates on fixed-point
+                                     records in FP32, the FP64 predicates deciding within its error
+                                     margins; the FP64 sweep where the margins do not hold);
+                                     1: LDS-staged cell runs (cell_divisor 1); 2: per-slot FP64 stencil
+                                     sweep; 3: LDS windows with an FP32 prefilter (cell_divisor 1-3);
+                                     4: LDS-staged stencil rows, one wavefront per 64 slots; every mode
+                                     writes the same lists */
     int32_t reserved;             /* must be 0 */
     int64_t N;
     int64_t maxneighbours;        /* initialize_neighbor_struct kwarg (200) */
