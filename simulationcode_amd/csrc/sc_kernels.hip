@@ -2974,6 +2974,31 @@ enum {
     FIN_SET_TARGET      // target = step + arg (arg >= 0), else none
 };
 
+// Sums of K tile-partial slots over a replica's nt tiles: thread t adds the
+// tiles t, t + SC_TB, t + 2 SC_TB, ... in that order (the order fixes the
+// bits), with the loads of up to eight such tiles issued before their adds
+// (the partials were written by blocks on every XCD and come from memory).
+template <int K>
+__device__ inline void tile_sums(const SC_AS1 double *P, const int (&slot)[K], int nt, double (&a)[K])
+{
+    constexpr int U = 8;
+    for (int q0 = threadIdx.x; q0 < nt; q0 += U * SC_TB) {
+        double v[U][K];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int q = q0 + u * SC_TB;
+#pragma unroll
+            for (int c = 0; c < K; c++) v[u][c] = q < nt ? P[(size_t)slot[c] * nt + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (q0 + u * SC_TB >= nt) break;
+#pragma unroll
+            for (int c = 0; c < K; c++) a[c] += v[u][c];
+        }
+    }
+}
+
 __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, int op, int sysmask, double arg)
 {
     const int r = blockIdx.x;
@@ -3034,8 +3059,8 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     case FIN_FORCE1:
     case FIN_FORCE2: {
         double a[3] = {0.0, 0.0, 0.0};
-        for (int q = tid; q < R.ntiles; q += SC_TB)
-            for (int c = 0; c < 3; c++) a[c] += P[(PS_VEC + c) * R.ntiles + q];
+        const int slots[3] = {PS_VEC, PS_VEC + 1, PS_VEC + 2};
+        tile_sums<3>(P, slots, R.ntiles, a);
         block_sum<3>(a);
         if (tid == 0) {
             for (int c = 0; c < R.dims; c++) {
@@ -3070,7 +3095,8 @@ __global__ __launch_bounds__(SC_TB) void k_finish(const Rep *reps, int nreps, in
     case FIN_EKIN:
     case FIN_EKIN_RESCALE: {
         double a[1] = {0.0};
-        for (int q = tid; q < R.ntiles; q += SC_TB) a[0] += P[PS_SCAL * R.ntiles + q];
+        const int slots[1] = {PS_SCAL};
+        tile_sums<1>(P, slots, R.ntiles, a);
         block_sum<1>(a);
         if (tid == 0) {
             double ek = a[0] * (R.m / 2.0);

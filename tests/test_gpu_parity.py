@@ -566,6 +566,34 @@ def test_large_system_forces_match_oracle():
     assert np.all(np.abs(g["F"].sum(axis=0)) <= 1e-12 * np.abs(g["F"]).sum())
 
 
+def test_prefilter_build_equals_fp64_sweep_at_1e6():
+    """N = 10^6 (the large-system shape, default cell size): the prefiltered
+    build (mode 0) and the FP64 sweep (mode 2) give bitwise-equal forces from
+    the same relaxed state (the force sums follow the list order, so the
+    lists agree entry for entry) and bitwise-equal trajectories over 40 steps
+    with their rebuilds."""
+    N, dims = 10 ** 6, 3
+    op, r, D, v, rc = H.make_system("berthier", N, dims, seed=20261206, relax=False, maxneighbours=100,
+                                    kBT=0.2)
+    out = []
+    for mode in (0, 2):
+        ctx = make_ctx(op, rc, r, np.zeros_like(r), D, cell_divisor=0, build_mode=mode)
+        ctx.update_neighbor_lists()
+        ctx.minimize(100, 0.005, 0.02)
+        ctx.set_state(v=v)
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        F0 = ctx.get_state(r=False, v=False, F=True)["F"]
+        ctx.do_time_steps(40)
+        out.append((F0, ctx.get_state(), ctx.rebuild_count(), ctx.neighbor_stats()))
+        ctx.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for f in ("r", "v", "F"):
+        assert np.array_equal(out[0][1][f], out[1][1][f]), f
+    assert out[0][2] == out[1][2] >= 3
+    assert out[0][3] == out[1][3]
+
+
 @pytest.mark.parametrize("cell_divisor", [1, 2])
 def test_forced_rebuild_keeps_stored_forces(cell_divisor):
     """update_neighbor_lists! between steps leaves every particle's stored F
