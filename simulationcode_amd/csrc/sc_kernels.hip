@@ -1716,11 +1716,11 @@ __device__ void build_slot_lanes_fx(const Rep &R, int k, int l, unsigned long lo
 
 // R5 (per-slot form) over the planned tiles.  Launch shape: BS threads per
 // block, LPP lanes per slot; a work item is BS/LPP consecutive slots of one
-// tile.  The lists do not depend on the launch shape.  With one lane per
-// slot the register budget is held to five wavefronts per SIMD (96 VGPRs,
-// ten of them spilled to scratch).
+// tile.  The lists do not depend on the launch shape.  The tile and replica
+// descriptors are read as constant memory (scalar registers), so the
+// one-lane sweep fits 59 VGPRs, seven wavefronts per SIMD, with no spills.
 template <int BS, int LPP>
-__global__ __launch_bounds__(BS, LPP == 1 ? 5 : 1) void k_build(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
+__global__ __launch_bounds__(BS) void k_build(const Tile *tiles, const Rep *reps, const int *plan, const int *flag,
                                               int nreps, int ntiles)
 {
     constexpr int SPI = BS / LPP;      // slots per work item
