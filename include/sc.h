@@ -256,7 +256,10 @@ int sc_batch_set_ahead(sc_batch *b, int32_t chunk);
 /* sc_batch_step in two halves: start the steps and return at once (several
  * batches then run concurrently on their own streams; the chunked loop runs
  * on a host thread of the library), and wait for them (device time, step
- * counters, error check).  Any other call on the batch waits first. */
+ * counters, error check).  Any other call on the batch waits first, except
+ * sc_batch_last_error, whose message the steps may still set: read it after
+ * sc_batch_wait.  Calls on one batch from several host threads at once are
+ * not supported (different batches are independent). */
 int sc_batch_step_async(sc_batch *b, int64_t nsteps);
 int sc_batch_wait(sc_batch *b);
 /* Capped steepest descent (sc_minimize) of every admitted replica. */

@@ -316,6 +316,10 @@ int sc_batch_add(sc_batch *b, const sc_params *p, const double *D, const double 
 {
     if (!b || !p || !D || !r || !id) return SC_ERR_ARG;
     if (!sc::params_valid(p)) return bfail(b, SC_ERR_ARG, "invalid parameters");
+    if (b->pending >= 0) {  // an asynchronous step's worker reads the slots: settle it first
+        int rc = sc_batch_wait(b);
+        if (rc) return rc;
+    }
     int slot = -1;
     for (int i = 0; i < b->max_reps; i++)
         if (!b->slots[i].used) {
