@@ -159,3 +159,30 @@ def test_chunked_ahead_equals_eager_stepping():
             assert np.array_equal(se[k][f], sa[k][f]), (k, f)
         assert te[k]["rebuilds"] == ta[k]["rebuilds"] and te[k]["steps"] == ta[k]["steps"] == 355
     assert sum(t["rebuilds"] for t in ta) > 3 * len(specs)
+
+
+def test_admission_during_an_async_step_waits_for_it():
+    """sc_batch_add while a chunked asynchronous step is in flight settles
+    that step first: the replicas end bitwise as with a synchronous step
+    before the admission, and the admitted replica steps as it would alone."""
+    first, late = SPECS[:3], SPECS[3]
+    out = []
+    for asynchronous in (False, True):
+        b = nat.Batch(16)
+        ids = [b.add(*W.replica(**s)) for s in first]
+        b.minimize(*RELAX)
+        if asynchronous:
+            b.step_async(90)
+        else:
+            b.step(90)
+        p, D, r, v = W.replica(**late)
+        ids.append(b.add(p, D, r, v))
+        b.minimize(*RELAX)  # the admitted lattice start (the same in both orders)
+        b.step(40)
+        out.append(([b.get_state(i) for i in ids], [b.stats(i)["steps"] for i in ids]))
+        b.close()
+    (sa, na), (sb, nb) = out
+    assert na == nb == [130, 130, 130, 40]
+    for k in range(len(sa)):
+        for f in ("r", "v", "F"):
+            assert np.array_equal(sa[k][f], sb[k][f]), (k, f)
