@@ -292,7 +292,10 @@ int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *dev
  * No reference counterpart: the reference's rebuild is CellListMap's CPU
  * cell list (src/NeighborLists.jl:46-52).  The per-slot neighbour-list build
  * kernel runs with block_size threads per block (64 or 256) and
- * lanes_per_particle lanes cooperating on one slot (1, 2, 4, 8, 16 or 32).
+ * lanes_per_particle lanes cooperating on one slot (1, 2, 4, 8, 16 or 32);
+ * block_size 256 with lanes_per_particle 0 selects the per-slot radius
+ * form (one lane per slot, searching only out to the slot's own largest
+ * half-list radius; the default of a single system).
  * The lists, and so every result, are identical for every shape; only the
  * time differs (one lane per slot suits one large system, several lanes
  * suit batches of small replicas whose rebuild is latency-bound). */
@@ -302,7 +305,7 @@ typedef struct sc_tune_entry {
     double ms_per_step; /* device time per step measured for this shape */
 } sc_tune_entry;
 
-#define SC_TUNE_SHAPES 12
+#define SC_TUNE_SHAPES 13
 
 int sc_set_build_launch(sc_ctx *ctx, int32_t block_size, int32_t lanes_per_particle);
 int sc_batch_set_build_launch(sc_batch *b, int32_t block_size, int32_t lanes_per_particle);

@@ -62,7 +62,7 @@ class TuneEntry(ctypes.Structure):
                 ("ms_per_step", ctypes.c_double)]
 
 
-TUNE_SHAPES = 12
+TUNE_SHAPES = 13
 
 
 def _tune(fn, h, steps):

@@ -167,15 +167,17 @@ struct RebuildGeom {
     int *plan = nullptr;     // device work list, plan_ints(nreps, ntiles, nwin) ints (k_rebuild_plan)
     const int *flag = nullptr;  // one replica without a plan: its RepDyn::rebuild on the device
     int build_bs = 256;      // per-slot build launch shape: threads per block (64 or 256)
-    int build_lpp = 1;       //   and lanes per slot (1 to 32, powers of two); the lists do not depend on it
+    int build_lpp = 0;       //   and lanes per slot (1 to 32, powers of two; 0 with 256 threads: the
+                             //   per-slot radius form); the lists do not depend on it
 };
 bool build_shape_valid(int bs, int lpp);
 // rebuild work list: header {tiles, replicas, window runs, 0}, then the
 // replicas, the tiles and the (replica, run) pairs of the rebuilding replicas
 inline int plan_ints(int nreps, int ntiles, int nwin) { return 4 + nreps + ntiles + 2 * nwin; }
-// the per-slot build launch shapes tried by the autotuner, {block, lanes}
-constexpr int kBuildShapes[12][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {256, 16}, {256, 32},
-                                     {64, 1},  {64, 2},  {64, 4},  {64, 8},  {64, 16},  {64, 32}};
+// the per-slot build launch shapes tried by the autotuner, {block, lanes};
+// lanes 0: the per-slot radius form (k_build_px)
+constexpr int kBuildShapes[13][2] = {{256, 1}, {256, 2}, {256, 4}, {256, 8}, {256, 16}, {256, 32}, {64, 1},
+                                     {64, 2},  {64, 4},  {64, 8},  {64, 16}, {64, 32},  {256, 0}};
 // bookkeep=false when the next launch is a velocity-Verlet force finish,
 // which does the rebuild bookkeeping itself
 int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, const RebuildGeom &g, hipStream_t s,

@@ -930,11 +930,11 @@ __device__ inline int rebuild_rep(const int *plan, const int *flag, int idx)
     return (idx == 0 && *gp(flag) != 0) ? 0 : -1;
 }
 
-// Slot range start of cell c: chunk-local exclusive prefix + chunk offset
-// (two-level scan), c == ncell gives n.
+// Slot range start of cell c (the scan leaves absolute starts); c == ncell
+// gives n.
 __device__ inline int cstart(const Rep &R, int c)
 {
-    return c >= R.ncell ? R.n : gp(R.cell_start)[c] + gp(R.chunk_off)[c >> SC_CHUNK_SHIFT];
+    return c >= R.ncell ? R.n : gp(R.cell_start)[c];
 }
 
 // XCD-aware grid-stride iteration for capped grids: the blocks resident on
@@ -1018,10 +1018,11 @@ __global__ __launch_bounds__(SC_TB) void k_bin(const Tile *tiles, const Rep *rep
     }
 }
 
-// R2a: chunk-local exclusive scan of the cell counts (SC_CHUNK cells per
-// block, 4 per thread); clears the counts for the next rebuild.  fused: the
-// block also sums the particle counts of the chunks before its own (counted
-// by k_bin) and stores its chunk's offset directly, so R2b is not launched.
+// R2a: exclusive scan of the cell counts (SC_CHUNK cells per block, 4 per
+// thread); clears the counts for the next rebuild.  fused: the block also
+// sums the particle counts of the chunks before its own (counted by k_bin)
+// and stores absolute cell starts directly, so R2b and R2c are not
+// launched; otherwise it stores chunk-local starts and its chunk's total.
 __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int *plan, const int *flag, int fused)
 {
     const int r = rebuild_rep(plan, flag, (int)blockIdx.y);
@@ -1030,12 +1031,13 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
     const int c0 = blockIdx.x * SC_CHUNK;
     if (c0 >= R.ncell) return;
     const int tid = threadIdx.x;
+    int base = 0;
     if (fused) {
         double before = 0.0;  // exact: counts are integers far below 2^53
         for (int q = tid; q < (int)blockIdx.x * SC_CHUNK_SPLIT; q += SC_TB) before += (double)gp(R.chunk_cnt)[q];
         double red[1] = {before};
         block_sum<1>(red);
-        if (tid == 0) gp(R.chunk_off)[blockIdx.x] = (int)red[0];
+        base = (int)red[0];
     }
     int v[4];
     int s = 0;
@@ -1054,7 +1056,7 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
         sums[tid] += x;
         __syncthreads();
     }
-    int run = sums[tid] - s;
+    int run = base + sums[tid] - s;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const int c = c0 + tid * 4 + q;
@@ -1065,6 +1067,19 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
         run += v[q];
     }
     if (!fused && tid == SC_TB - 1) gp(R.chunk_off)[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
+}
+
+// R2c (after R2b, unfused scans only): chunk offsets added to the
+// chunk-local starts, so that every cell start is absolute.
+__global__ __launch_bounds__(SC_TB) void k_scan_fold(const Rep *reps, const int *plan, const int *flag)
+{
+    const int r = rebuild_rep(plan, flag, (int)blockIdx.y);
+    if (r < 0) return;
+    const Rep R = reps[r];
+    const int c0 = blockIdx.x * SC_CHUNK;
+    if (c0 >= R.ncell) return;
+    const int off = gp(R.chunk_off)[blockIdx.x];
+    for (int c = c0 + (int)threadIdx.x; c < min(c0 + SC_CHUNK, R.ncell); c += SC_TB) gp(R.cell_start)[c] += off;
 }
 
 // R2b: exclusive scan of the chunk totals (one block per replica).
@@ -1177,12 +1192,12 @@ __device__ inline void build_one(const Rep &R, int k, const double4 p, int &nful
     }
 }
 
+// The permuted state of slot k moved back from the _alt buffers (p: its
+// position, pos_alt[k]).
 template <int DIMS>
-__device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, int nfull, int cnt)
+__device__ inline void build_copy_state(const Rep &R, int k, const double4 p)
 {
     const int n = R.n;
-    if (nfull > R.cap) gatomic_or(&gp(R.dyn)->err, SC_EBIT_OVERFLOW);
-    gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
     st4(gp(R.pos) + (k), p);
     put_shadow(R, k, p);
     const bool keepF = (gp(R.dyn)->rebuild & REBUILD_KEEP_F) != 0;
@@ -1192,6 +1207,14 @@ __device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, i
         if (keepF) gp(R.F)[c * n + k] = gp(R.stage)[c * n + k];
     }
     gp(R.orig)[k] = gp(R.orig_alt)[k];
+}
+
+template <int DIMS>
+__device__ inline void build_finish_slot(const Rep &R, int k, const double4 p, int nfull, int cnt)
+{
+    if (nfull > R.cap) gatomic_or(&gp(R.dyn)->err, SC_EBIT_OVERFLOW);
+    gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
+    build_copy_state<DIMS>(R, k, p);
 }
 
 // LPP lanes cooperate on one particle: each takes every LPP-th candidate of
@@ -1780,6 +1803,238 @@ __global__ __launch_bounds__(BS) void k_build_fx(const Tile *tiles, const Rep *r
             else
                 build_slot_lanes_fx<2, LPP>(R, k, l, gmask, below);
         }
+    }
+}
+
+// R5, per-slot radius form (launch shape {256, 0}; the replicas with
+// R.bf_on).  The lists are entry for entry those of build_slot_fx: the same
+// candidate order, the same prefilter and FP64 predicates.  What changes is
+// which candidates are visited and how a lane walks them.
+//  * Search radius per slot.  Every half-list radius of slot k is at most
+//    R_k = rcut * max_j sigma_kj + skin over all diameters D_j in [dmin, dmax]:
+//    sigma_kj = (D_k + D_j)/2 (1 - zeta |D_k - D_j|) grows with D_j below
+//    D_k, and above it while D_j < 1/(2 zeta), so the maximum is at
+//    D_j = min(dmax, max(D_k, 1/(2 zeta))).  The stencil rows are cut to
+//    min(R_k, R_v) instead of R_v.  The candidates left out are farther than
+//    every half-list radius of the slot, so no accepted entry is lost.
+//  * Overflow.  The full-list count (pairs within R_v) is bounded by the
+//    particles in the stencil rows cut to R_v, which the cell offsets give
+//    without visiting them.  A slot whose bound exceeds the row capacity
+//    counts its full-list pairs exactly in one more sweep, so the overflow
+//    bit is raised exactly when some full-list row is longer than the cap.
+//  * Flattened rows.  Each lane lists the slot ranges of its next stencil
+//    rows in LDS (up to PX_TMAX parts) and walks them as one stream with four
+//    records in flight, so a wavefront runs as long as its longest stream,
+//    not the sum over rows of the longest row.  A row's cell offsets are
+//    loaded together, and the next row's while a row is listed.
+#ifndef SC_PX_TMAX
+#define SC_PX_TMAX 12
+#endif
+#ifndef SC_PX_U
+#define SC_PX_U 4
+#endif
+constexpr int PX_TMAX = SC_PX_TMAX;  // row parts listed per lane before a sweep
+constexpr int PX_U = SC_PX_U;        // candidate records in flight per lane
+
+struct PxLds {
+    int2 tab[PX_TMAX][SC_TB];  // part e of thread t at [e][t]
+};
+
+// Upper bound on every half-list radius of a particle of diameter Dk.
+__device__ inline double px_radius(const Rep &R, double Dk)
+{
+    if (R.pot != 2) return R.rcut * R.lj_md + R.skin;
+    const double dmax = fmax((double)R.dmax, Dk);
+    const double Dj = R.zeta > 0.0 ? fmin(dmax, fmax(Dk, 0.5 / R.zeta)) : dmax;
+    const double md = (Dk + Dj) * 0.5 * (1.0 - R.zeta * fabs(Dk - Dj));
+    // 1e-5 relative: R.dmax is the FP32 rounding of the largest diameter
+    return (R.rcut * md + R.skin) * (1.0 + 1e-5);
+}
+
+// Cell ids bounding the slot ranges of the raw cells [x0, x1] of the cell
+// row starting at id `row`, split where the row wraps the box, in the
+// cell-sweep order of for_stencil_ranges: part p is [cstart(id[2p]),
+// cstart(id[2p+1])); ids stay -1 where a part is absent.
+__device__ __forceinline__ void px_run_ids(int row, int x0, int x1, int nc, int *id)
+{
+    if (x0 > x1) return;
+    if (x0 < 0) {
+        id[0] = row + x0 + nc;
+        id[1] = row + min(x1, -1) + nc + 1;
+        if (x1 >= 0) {
+            id[2] = row;
+            id[3] = row + x1 + 1;
+        }
+    } else if (x1 >= nc) {
+        if (x0 <= nc - 1) {
+            id[0] = row + x0;
+            id[1] = row + nc;
+        }
+        id[2] = row + max(x0, nc) - nc;
+        id[3] = row + x1 - nc + 1;
+    } else {
+        id[0] = row + x0;
+        id[1] = row + x1 + 1;
+    }
+}
+
+// x run [lo, hi] of a stencil row at squared y/z gap g2 within radius^2 r2
+// (cell units, with the slack of for_stencil_ranges); lo > hi when empty.
+__device__ __forceinline__ void px_run(float r2, float g2, float fx, int s, int &lo, int &hi)
+{
+    lo = 1;
+    hi = 0;
+    if (g2 > r2) return;
+    const float rx = sqrtf(r2 - g2) + 1e-4f;
+    lo = max(-s, (int)floorf(fx - rx));
+    hi = min(s, (int)floorf(fx + rx));
+}
+
+// Walks the ne listed parts of this lane as one stream of candidates.
+template <int DIMS>
+__device__ __forceinline__ void px_sweep(const Rep &R, int k, const uint4 &pk, float sk, int ne, const PxLds &S,
+                                         int &cnt)
+{
+    const int tid = threadIdx.x;
+    int ei = 0, j = 0, e = 0;
+    if (ne > 0) {
+        const int2 t = S.tab[0][tid];
+        j = t.x;
+        e = t.y;
+    }
+    while (__ballot(j < e) != 0ull) {
+        int jj[PX_U];
+        bool ok[PX_U];
+#pragma unroll
+        for (int u = 0; u < PX_U; u++) {
+            ok[u] = j < e;
+            jj[u] = ok[u] ? j : k;
+            j++;
+            if (j >= e && ei + 1 < ne) {
+                ei++;
+                const int2 t = S.tab[ei][tid];
+                j = t.x;
+                e = t.y;
+            }
+        }
+        uint4 q[PX_U];
+#pragma unroll
+        for (int u = 0; u < PX_U; u++) q[u] = ok[u] ? ldq(R.bpos + jj[u]) : pk;  // no load past a lane's stream
+#pragma unroll
+        for (int u = 0; u < PX_U; u++) {
+            if (!ok[u] || jj[u] == k) continue;
+            const int res = prefilter_q<DIMS>(R, k, pk, sk, jj[u], q[u]);
+            if (res & 2) {
+                if (cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = jj[u];
+                cnt++;
+            }
+        }
+    }
+}
+
+template <int DIMS>
+__device__ void build_tile_px(const Rep &R, const Tile &t, PxLds &S)
+{
+    const int tid = threadIdx.x, n = R.n;
+    const float cs = (float)(R.cinv * R.cinv);
+    const float rv2c = (float)(R.Rv2 * R.cinv * R.cinv) * (1.0f + 1e-4f) + 1e-4f;
+    // state copy-back in slot order
+    const int k = t.first + tid;
+    const bool have = k < n;  // lanes without a slot join the wavefront's ballots only
+    if (have) build_copy_state<DIMS>(R, k, ld4(gp(R.pos_alt) + (k)));
+    int c0 = 0, c1 = 0, c2 = 0;
+    float fx = 0.f, fy = 0.f, fz = 0.f, sk = 0.f, rs2c = 0.f;
+    uint4 pk = make_uint4(0u, 0u, 0u, 0u);
+    if (have) {
+        slot_cell<DIMS>(R, k, c0, c1, c2, fx, fy, fz);
+        pk = ldq(R.bpos + k);
+        sk = __uint_as_float(pk.w);
+        const double rk = fmin(px_radius(R, ld4(gp(R.pos_alt) + (k)).w), sqrt(R.Rv2));
+        rs2c = (float)(rk * rk) * cs * (1.0f + 1e-4f) + 1e-4f;
+    }
+    const int s = R.srange, nc = R.nc, side = 2 * s + 1, nrows = side * (DIMS == 3 ? side : 1);
+    // Stencil row r (oz outer, oy inner): the slot-range bounds of its parts
+    // as cell offsets, v[0..3] for the run cut to the search radius and
+    // v[4..7] for the run cut to R_v (empty parts {0, 0}; same: the two runs
+    // coincide and v[4..7] is not loaded).  All of a row's offsets are
+    // loaded before any is used, and the next row's are in flight while a
+    // row is listed.
+    auto fetch_row = [&](int r, int (&v)[8], bool &same) __attribute__((always_inline)) {
+        int id[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) id[q] = -1;
+        same = true;
+        if (have) {
+            const int oz = DIMS == 3 ? r / side - s : 0, oy = r % side - s;
+            const float gz = DIMS == 3 ? cell_gap32(oz, fz) : 0.0f, gy = cell_gap32(oy, fy);
+            const float g2 = gy * gy + gz * gz;
+            int lov, hiv, los, his;
+            px_run(rv2c, g2, fx, s, lov, hiv);
+            px_run(rs2c, g2, fx, s, los, his);
+            const int row = nc * (wrapc(c1 + oy, nc) + nc * (DIMS == 3 ? wrapc(c2 + oz, nc) : 0));
+            px_run_ids(row, c0 + los, c0 + his, nc, id);
+            same = lov == los && hiv == his;
+            if (!same) px_run_ids(row, c0 + lov, c0 + hiv, nc, id + 4);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = id[q] >= 0 ? cstart(R, id[q]) : 0;
+    };
+    int bound = -1, cnt = 0, ne = 0;  // bound: particles in the R_v rows, self excluded
+    int vc[8];
+    bool samec;
+    fetch_row(0, vc, samec);
+    for (int r = 0; r < nrows; r++) {
+        int vn[8];
+        bool samen = true;
+        if (r + 1 < nrows) fetch_row(r + 1, vn, samen);
+        if (have) {
+            int listed = 0;
+#pragma unroll
+            for (int p = 0; p < 2; p++) {
+                const int b = vc[2 * p], e = vc[2 * p + 1];
+                if (b < e) {
+                    S.tab[ne][tid] = make_int2(b, e);
+                    ne++;
+                    listed += e - b;
+                }
+            }
+            bound += samec ? listed : (vc[5] - vc[4]) + (vc[7] - vc[6]);
+        }
+        if (r == nrows - 1 || __ballot(ne > PX_TMAX - 3) != 0ull) {
+            px_sweep<DIMS>(R, have ? k : t.first, pk, sk, ne, S, cnt);
+            ne = 0;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) vc[q] = vn[q];
+        samec = samen;
+    }
+    if (!have) return;
+    if (bound > R.cap) {
+        // the bound allows an overflow: count the full-list pairs exactly
+        int nfull = 0;
+        for_stencil_ranges<DIMS>(R, c0, c1, c2, fx, fy, fz, [&](int b, int e) __attribute__((always_inline)) {
+            for (int j = b; j < e; j++)
+                if (j != k) nfull += prefilter_pair<DIMS>(R, k, pk, sk, j) & 1;
+        });
+        if (nfull > R.cap) gatomic_or(&gp(R.dyn)->err, SC_EBIT_OVERFLOW);
+    }
+    gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
+}
+
+__global__ __launch_bounds__(SC_TB) void k_build_px(const Tile *tiles, const Rep *reps, const int *plan,
+                                                     const int *flag, int nreps, int ntiles)
+{
+    __shared__ PxLds S;
+    const RebuildWork wk = rebuild_work(plan, nreps, flag, ntiles);
+    for (XcdRange it = xcd_range(wk.count); it.cur < it.end; it.cur += it.step) {
+        const Tile t = ldc(tiles, wk.tile(it.cur));
+        const Rep R = ldc(reps, t.rep);
+        if (R.build_kind != BUILD_SLOT || !R.bf_on) continue;
+        if (R.dims == 3)
+            build_tile_px<3>(R, t, S);
+        else
+            build_tile_px<2>(R, t, S);
+        __syncthreads();  // the LDS tables are reused by the next tile
     }
 }
 
@@ -3284,6 +3539,7 @@ int launch_gather_err(const Rep *reps, int nreps, int *out, hipStream_t s, int w
 
 bool build_shape_valid(int bs, int lpp)
 {
+    if (lpp == 0) return bs == 256;  // the per-slot radius form
     return (bs == 64 || bs == 256) && (lpp == 1 || lpp == 2 || lpp == 4 || lpp == 8 || lpp == 16 || lpp == 32);
 }
 
@@ -3313,7 +3569,10 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     const int fused = g.max_chunks <= SC_FUSED_CHUNKS ? 1 : 0;
     hipLaunchKernelGGL(k_bin, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles, fused);
     hipLaunchKernelGGL(k_scan_local, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan, flag, fused);
-    if (!fused) hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan, flag);
+    if (!fused) {
+        hipLaunchKernelGGL(k_scan_top, dim3(nreps), dim3(SC_TB), 0, s, reps, plan, flag);
+        hipLaunchKernelGGL(k_scan_fold, dim3(g.max_chunks, nreps), dim3(SC_TB), 0, s, reps, plan, flag);
+    }
     hipLaunchKernelGGL(k_scatter, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles, fused);
     hipLaunchKernelGGL(k_rank_permute, dim3(gt), dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
     if (g.nruns > 0) {
@@ -3335,6 +3594,13 @@ int launch_rebuild(const Tile *tiles, int ntiles, const Rep *reps, int nreps, co
     if (g.any_tile_build) {
         const int key = g.build_bs * 64 + g.build_lpp, w = g.any_tile_build;
         switch (key) {
+        case 256 * 64 + 0: {
+            // per-slot radius form for the prefiltered replicas, the FP64 sweep for the others
+            const dim3 grid(ntiles < SC_GRID_CAP ? ntiles : SC_GRID_CAP);
+            if (w & 1) launch_build_shape<256, 1>(tiles, ntiles, reps, nreps, plan, flag, s, 1);
+            if (w & 2) hipLaunchKernelGGL(k_build_px, grid, dim3(SC_TB), 0, s, tiles, reps, plan, flag, nreps, ntiles);
+            break;
+        }
         case 64 * 64 + 1: launch_build_shape<64, 1>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
         case 64 * 64 + 2: launch_build_shape<64, 2>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;
         case 64 * 64 + 4: launch_build_shape<64, 4>(tiles, ntiles, reps, nreps, plan, flag, s, w); break;

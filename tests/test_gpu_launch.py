@@ -11,7 +11,7 @@ from simulationcode_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(256, 1), (256, 2), (256, 4), (256, 8), (256, 16), (256, 32),
-          (64, 1), (64, 2), (64, 4), (64, 8), (64, 16), (64, 32)]
+          (64, 1), (64, 2), (64, 4), (64, 8), (64, 16), (64, 32), (256, 0)]
 
 CASES = [
     dict(N=3000, dims=3, system="newtonian", dt=0.005, precision="fp64", cell_divisor=2, seed=11),
@@ -56,6 +56,8 @@ def test_invalid_shape_rejected():
         ctx.set_build_launch(128, 1)
     with pytest.raises(nat.ScError):
         ctx.set_build_launch(256, 3)
+    with pytest.raises(nat.ScError):
+        ctx.set_build_launch(64, 0)
     ctx.close()
 
 

@@ -19,8 +19,10 @@ pytestmark = pytest.mark.gpu
 CASES = [("lj", 3, 1000), ("berthier", 3, 512), ("berthier", 2, 900), ("lj", 2, 400), ("berthier", 3, 125)]
 
 
-def make_ctx(op, rc, r, v, D, precision=nat.PREC_FP64, cell_divisor=1, seed=1234, build_mode=0):
+def make_ctx(op, rc, r, v, D, precision=nat.PREC_FP64, cell_divisor=1, seed=1234, build_mode=0, shape=None):
     ctx = nat.Context(H.sc_params_from(op, rc, precision, cell_divisor, seed, build_mode), D)
+    if shape is not None:
+        ctx.set_build_launch(*shape)
     ctx.set_state(r=r, v=v)
     return ctx
 
@@ -254,17 +256,20 @@ def test_row_build_lists_equal_per_slot_lists(kind, dims, N, cell_divisor):
 @pytest.mark.parametrize("kind,dims,N", [("berthier", 3, 6000), ("berthier", 2, 5000), ("lj", 3, 4000),
                                          ("lj", 2, 3000)])
 @pytest.mark.parametrize("cell_divisor", [1, 2, 3])
-def test_prefilter_build_equals_fp64_sweep(kind, dims, N, cell_divisor):
+@pytest.mark.parametrize("shape", [(256, 1), (256, 0)], ids=["rv", "radius"])
+def test_prefilter_build_equals_fp64_sweep(kind, dims, N, cell_divisor, shape):
     """The automatic per-slot build (mode 0) tests each candidate on its
     fixed-point record in FP32 within proven error margins and decides the
     pairs inside the margins with the FP64 predicates; its lists equal the
     FP64-only sweep's (mode 2) entry for entry, so the trajectories agree
     bitwise, also over the rebuilds of 60 steps and a rebuild requested
-    between steps."""
+    between steps.  Both one-lane forms: the sweep to R_v and the sweep
+    to each slot's own largest half-list radius."""
     op, r, D, v, rc = H.make_system(kind, N, dims, seed=120 + cell_divisor + dims)
     out = []
     for mode in (0, 2):
-        ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=mode)
+        ctx = make_ctx(op, rc, r, v, D, cell_divisor=cell_divisor, build_mode=mode,
+                       shape=shape if mode == 0 else None)
         ctx.update_neighbor_lists()
         first = ctx.get_neighbor_lists()
         ctx.do_time_steps(60)
@@ -414,18 +419,20 @@ def test_overflow_raises():
         ctx.update_neighbor_lists()
 
 
-@pytest.mark.parametrize("cell_divisor,build_mode", [(1, 3), (2, 3), (2, 4)])
-def test_overflow_threshold_exact(cell_divisor, build_mode):
+@pytest.mark.parametrize("cell_divisor,build_mode,shape", [(1, 3, None), (2, 3, None), (2, 4, None),
+                                                          (1, 0, (256, 0)), (2, 0, (256, 0)), (3, 0, (256, 0)),
+                                                          (2, 0, (256, 1))])
+def test_overflow_threshold_exact(cell_divisor, build_mode, shape):
     """maxneighbours equal to the longest full row builds; one less is an
-    overflow (the windowed build counts full rows exactly whenever its
-    candidate bound exceeds the capacity)."""
+    overflow (the windowed and the per-slot radius builds count full rows
+    exactly whenever their candidate bound exceeds the capacity)."""
     op, r, D, v, rc = H.make_system("berthier", 4000, 3, seed=12)
     M = int(oracle.build_lists(op, r, D, cells=True).nfull.max())
     for cap, raises in ((M, False), (M - 1, True)):
         op2 = oracle.make_params(dims=3, N=4000, box_size=op.box_size, force_cutoff=rc, skin=op.skin_distance,
                                  D=D, potential=oracle.POT_BERTHIER, system=oracle.SYS_NEWTONIAN,
                                  kBT=0.15, gamma=1.0, dt=0.005, maxneighbours=cap)
-        ctx = make_ctx(op2, rc, r, v, D, cell_divisor=cell_divisor, build_mode=build_mode)
+        ctx = make_ctx(op2, rc, r, v, D, cell_divisor=cell_divisor, build_mode=build_mode, shape=shape)
         if raises:
             with pytest.raises(nat.NeighborOverflowError):
                 ctx.update_neighbor_lists()
@@ -571,13 +578,13 @@ def test_prefilter_build_equals_fp64_sweep_at_1e6():
     build (mode 0) and the FP64 sweep (mode 2) give bitwise-equal forces from
     the same relaxed state (the force sums follow the list order, so the
     lists agree entry for entry) and bitwise-equal trajectories over 40 steps
-    with their rebuilds."""
+    with their rebuilds; both one-lane forms of the prefiltered build."""
     N, dims = 10 ** 6, 3
     op, r, D, v, rc = H.make_system("berthier", N, dims, seed=20261206, relax=False, maxneighbours=100,
                                     kBT=0.2)
     out = []
-    for mode in (0, 2):
-        ctx = make_ctx(op, rc, r, np.zeros_like(r), D, cell_divisor=0, build_mode=mode)
+    for mode, shape in ((2, None), (0, (256, 1)), (0, (256, 0))):
+        ctx = make_ctx(op, rc, r, np.zeros_like(r), D, cell_divisor=0, build_mode=mode, shape=shape)
         ctx.update_neighbor_lists()
         ctx.minimize(100, 0.005, 0.02)
         ctx.set_state(v=v)
@@ -587,11 +594,12 @@ def test_prefilter_build_equals_fp64_sweep_at_1e6():
         ctx.do_time_steps(40)
         out.append((F0, ctx.get_state(), ctx.rebuild_count(), ctx.neighbor_stats()))
         ctx.close()
-    assert np.array_equal(out[0][0], out[1][0])
-    for f in ("r", "v", "F"):
-        assert np.array_equal(out[0][1][f], out[1][1][f]), f
-    assert out[0][2] == out[1][2] >= 3
-    assert out[0][3] == out[1][3]
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0])
+        for f in ("r", "v", "F"):
+            assert np.array_equal(out[0][1][f], o[1][f]), f
+        assert out[0][2] == o[2] >= 3
+        assert out[0][3] == o[3]
 
 
 @pytest.mark.parametrize("cell_divisor", [1, 2])
