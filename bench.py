@@ -211,6 +211,9 @@ def bench_large(args, nat, synth):
     log(f"[bench] setup N={N} L={L:.3f}")
     cc = None if args.no_copy_ceiling else nat.copy_bandwidth()  # streaming ceiling, 1 GiB copies
     ctx, p = make_ctx(nat, N, L, D, nat.PREC_FP64, cell_divisor=args.cell_divisor, build_mode=args.build_mode)
+    shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
+    if shape:
+        ctx.set_build_launch(*shape)
     ctx.set_state(r=r, v=np.zeros((N, dims)), F=np.zeros((N, dims)))
     ctx.update_neighbor_lists()
     t0 = time.perf_counter()
@@ -270,6 +273,8 @@ def bench_large(args, nat, synth):
 
     if not args.no_mixed:
         ctxm, _ = make_ctx(nat, N, L, D, nat.PREC_MIXED, cell_divisor=args.cell_divisor, build_mode=args.build_mode)
+        if shape:
+            ctxm.set_build_launch(*shape)
         ctxm.set_state(r=state["r"], v=state["v"], F=state["F"])
         ctxm.update_neighbor_lists()
         ctx.set_state(r=state["r"], v=state["v"], F=state["F"])
@@ -496,7 +501,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261206)
     ap.add_argument("--cell-divisor", type=int, default=0,
                     help="0: library default (large) / autotuned (batched)")
-    ap.add_argument("--shape", default="", help="batched with --cell-divisor: build launch shape BS,LPP")
+    ap.add_argument("--shape", default="",
+                    help="build launch shape BS,LPP (large: both contexts; batched: with --cell-divisor)")
     ap.add_argument("--build-mode", type=int, default=0, help="large: sc_params.build_mode (0 automatic)")
     ap.add_argument("--chunk", type=int, default=8,
                     help="batched: steps launched ahead per chunk (0: one launch sequence per step)")

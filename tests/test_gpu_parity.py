@@ -203,6 +203,36 @@ def test_trajectory_rescale_temperature():
     assert np.max(np.abs(g["v"] - st.v)) <= 1e-10
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_rescale_velocities_matches_oracle(dims):
+    """rescale_velocities! (src/MD.jl:391-402) on the device: the same scaled
+    velocities as the oracle on the same input (within a few ulp: the kinetic
+    energy is summed in another order), kinetic temperature at kBT afterwards,
+    and positions untouched; also after steps have left a COM shift pending."""
+    op, r, D, v, rc = H.make_system("berthier", 900, dims, seed=31 + dims, kBT=0.25)
+    v = v * 1.7 + 0.05  # off the target temperature, with a COM velocity
+    ctx = make_ctx(op, rc, r, v, D)
+    ctx.update_neighbor_lists()
+    ctx.rescale_velocities()
+    g = ctx.get_state()
+    vo = np.ascontiguousarray(v.copy())
+    oracle.rescale_velocities(op, vo)
+    assert np.max(np.abs(g["v"] - vo)) <= 1e-14 * np.max(np.abs(vo))
+    assert np.array_equal(g["r"], r)
+    kT = np.sum(g["v"] ** 2) / (dims * len(v))  # m = 1
+    assert abs(kT - op.kBT) <= 1e-12 * op.kBT
+    # after ten steps (a COM shift is left pending by the last step's finish)
+    ctx.do_time_steps(10)
+    before = ctx.get_state()
+    vo = np.ascontiguousarray(before["v"].copy())
+    oracle.rescale_velocities(op, vo)
+    ctx.rescale_velocities()
+    after = ctx.get_state()
+    assert np.max(np.abs(after["v"] - vo)) <= 1e-14 * np.max(np.abs(vo))
+    assert np.array_equal(after["r"], before["r"])
+    ctx.close()
+
+
 @pytest.mark.parametrize("cell_divisor,build_mode", [(2, 0), (1, 1), (2, 3), (2, 4), (3, 4)])
 def test_trajectory_many_rebuilds(cell_divisor, build_mode):
     op, st, g, dr, ctx = _trajectory("berthier", 3, 1000, oracle.SYS_LANGEVIN, 400, dt=0.01,
