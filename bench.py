@@ -427,20 +427,22 @@ def W_replica(spec, cd):
     return W.replica(**dict(spec, cell_divisor=cd))
 
 
+BATCHED_CD, BATCHED_SHAPE = 2, (256, 0)  # measured best launch choices of the 256-replica batch
+
+
 def batched_summary(args, nat, steps=300, warmup=20):
     """The batched workload at a bounded length for the default run: the 256
-    replicas at the launch choices the full autotuned run settles on (cell
-    divisor 1, build 256 threads x 4 lanes per slot, two streams, chunks of 8
-    steps)."""
+    replicas at the best measured launch choices (cell divisor 2, the
+    per-slot radius build {256, 0}, two streams, chunks of 8 steps)."""
     from simulationcode_amd import workloads as W
     t0 = time.perf_counter()
     specs = W.batched_specs(args.replicas)
-    res = run_batched(nat, specs, 1, (256, 4), 2, False, warmup, steps)
+    res = run_batched(nat, specs, BATCHED_CD, BATCHED_SHAPE, 2, False, warmup, steps)
     res.update({"steps": steps, "warmup": warmup, "dt": 1e-4, "dynamics": "Brownian", "dtype": "mixed",
-                "cell_divisor": 1, "streams": 2, "wall_s": round(time.perf_counter() - t0, 1)})
+                "cell_divisor": BATCHED_CD, "streams": 2, "wall_s": round(time.perf_counter() - t0, 1)})
     log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step, {res['value']:.3g} particle-steps/s, "
         f"roofline {res['roofline']['frac']:.3f}, {res['rebuilds_per_step']:.2f} rebuilds/step")
-    res["stated_dt"] = sd = run_batched_stated_dt(nat, specs, 1, (256, 4))
+    res["stated_dt"] = sd = run_batched_stated_dt(nat, specs, BATCHED_CD, BATCHED_SHAPE)
     log(f"[bench] batched at dt=0.001: {sd['failed']}/{sd['replicas']} replicas non-finite within "
         f"{sd['steps']} steps (first at step {sd['failure_steps_p0_p50'][0] if sd['failed'] else '-'}), "
         f"{sd['rebuilds_per_step']} rebuilds/step, {sd['value']:.3g} particle-steps/s")
