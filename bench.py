@@ -34,9 +34,10 @@ median of three runs, once on one thread and once on all host cores.
 
 --workload batched: 256 independent replicas (N log-uniform in [500, 20000],
 75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian, per-replica
-seeds, mixed precision) split over two batches stepped concurrently on their
-own HIP streams, kernels launched eagerly (a captured hipGraph replay and a
-single stream were both measured slower); value = aggregate
+seeds, mixed precision) split over four batches stepped concurrently on their
+own HIP streams, steps launched ahead in chunks of 12 (a captured hipGraph
+replay, one to three or six to eight streams and chunks of 4-16 steps were all
+measured slower); value = aggregate
 particle-steps/s over the wall time of the timed steps.  dt = 1e-4: at the stated dt = 0.001 explicit
 Euler-Maruyama blows up within 10^2-10^3 steps for this potential.
 --workload served: see simulationcode_amd/serve.py.
@@ -427,19 +428,22 @@ def W_replica(spec, cd):
     return W.replica(**dict(spec, cell_divisor=cd))
 
 
-BATCHED_CD, BATCHED_SHAPE = 2, (256, 0)  # measured best launch choices of the 256-replica batch
+# measured best launch choices of the 256-replica batch
+BATCHED_CD, BATCHED_SHAPE, BATCHED_STREAMS, BATCHED_CHUNK = 2, (256, 0), 4, 12
 
 
 def batched_summary(args, nat, steps=300, warmup=20):
     """The batched workload at a bounded length for the default run: the 256
     replicas at the best measured launch choices (cell divisor 2, the
-    per-slot radius build {256, 0}, two streams, chunks of 8 steps)."""
+    per-slot radius build {256, 0}, four streams, chunks of 12 steps)."""
     from simulationcode_amd import workloads as W
     t0 = time.perf_counter()
     specs = W.batched_specs(args.replicas)
-    res = run_batched(nat, specs, BATCHED_CD, BATCHED_SHAPE, 2, False, warmup, steps)
+    res = run_batched(nat, specs, BATCHED_CD, BATCHED_SHAPE, BATCHED_STREAMS, False, warmup, steps,
+                      chunk=BATCHED_CHUNK)
     res.update({"steps": steps, "warmup": warmup, "dt": 1e-4, "dynamics": "Brownian", "dtype": "mixed",
-                "cell_divisor": BATCHED_CD, "streams": 2, "wall_s": round(time.perf_counter() - t0, 1)})
+                "cell_divisor": BATCHED_CD, "streams": BATCHED_STREAMS, "chunk": BATCHED_CHUNK,
+                "wall_s": round(time.perf_counter() - t0, 1)})
     log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step, {res['value']:.3g} particle-steps/s, "
         f"roofline {res['roofline']['frac']:.3f}, {res['rebuilds_per_step']:.2f} rebuilds/step")
     res["stated_dt"] = sd = run_batched_stated_dt(nat, specs, BATCHED_CD, BATCHED_SHAPE)
@@ -496,7 +500,7 @@ def main():
                     help="large: steps of each form's long window (at least five rebuild intervals)")
     ap.add_argument("--graph", action="store_true",
                     help="batched: replay the step as a captured hipGraph (eager launches measured faster)")
-    ap.add_argument("--streams", type=int, default=2,
+    ap.add_argument("--streams", type=int, default=4,
                     help="batched: split the replicas over this many batches on their own streams")
     ap.add_argument("--drift-steps", type=int, default=20000)
     ap.add_argument("--replicas", type=int, default=256)
@@ -506,7 +510,7 @@ def main():
     ap.add_argument("--shape", default="",
                     help="build launch shape BS,LPP (large: both contexts; batched: with --cell-divisor)")
     ap.add_argument("--build-mode", type=int, default=0, help="large: sc_params.build_mode (0 automatic)")
-    ap.add_argument("--chunk", type=int, default=8,
+    ap.add_argument("--chunk", type=int, default=12,
                     help="batched: steps launched ahead per chunk (0: one launch sequence per step)")
     ap.add_argument("--rate", type=float, nargs="*", default=None, help="served: arrival rates (jobs/s)")
     ap.add_argument("--jobs", type=int, default=48, help="served: jobs per arrival rate")
