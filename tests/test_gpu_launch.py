@@ -92,7 +92,7 @@ def batch_run(shape=None, tune_steps=0, steps=200):
 
 def test_batch_shapes_and_autotune_agree():
     ref, _, _ = batch_run(shape=(256, 1))
-    for shape in ((64, 8), (256, 4)):
+    for shape in ((64, 8), (256, 4), (256, 0)):
         got, _, _ = batch_run(shape=shape)
         for a, c in zip(ref, got):
             for f in ("r", "v", "F"):
