@@ -1890,6 +1890,34 @@ __device__ __forceinline__ void px_run(float r2, float g2, float fx, int s, int 
     hi = min(s, (int)floorf(fx + rx));
 }
 
+// The decisions of prefilter_q with one branch left (the margins' bands,
+// where the FP64 test decides): 0 out, 1 full list only, 3 full and half
+// list, 4 decide with exact_pair.  Every quantity is formed exactly as in
+// prefilter_q; those prefilter_q skips are computed and not used.
+template <int DIMS>
+__device__ __forceinline__ int prefilter_sel(const Rep &R, const uint4 &pk, float sk, const uint4 &q)
+{
+    const float fx = (float)(int)(pk.x - q.x);
+    const float fy = (float)(int)(pk.y - q.y);
+    float d2 = fx * fx + fy * fy;
+    if (DIMS == 3) {
+        const float fz = (float)(int)(pk.z - q.z);
+        d2 = d2 + fz * fz;
+    }
+    float lo = R.bf_rh2lo, hi = R.bf_rh2hi;
+    if (R.pot == 2) {
+        const float sj = __uint_as_float(q.w);
+        const float md = (sk + sj) * 0.5f * (1.0f - R.bf_zeta * fabsf(sk - sj));
+        const float rh = (R.bf_rcut * md + R.bf_skin) * R.bf_S;
+        const float rh2 = rh * rh;
+        lo = rh2 * R.bf_hlo - R.bf_m;
+        hi = rh2 * R.bf_hhi + R.bf_m;
+    }
+    const bool hin = d2 < lo;
+    const bool band = d2 >= R.bf_lo || (!hin && d2 <= hi);
+    return d2 > R.bf_hi ? 0 : (band ? 4 : (hin ? 3 : 1));
+}
+
 // Walks the ne listed parts of this lane as one stream of candidates.
 template <int DIMS>
 __device__ __forceinline__ void px_sweep(const Rep &R, int k, const uint4 &pk, float sk, int ne, const PxLds &S,
@@ -1922,12 +1950,11 @@ __device__ __forceinline__ void px_sweep(const Rep &R, int k, const uint4 &pk, f
         for (int u = 0; u < PX_U; u++) q[u] = ok[u] ? ldq(R.bpos + jj[u]) : pk;  // no load past a lane's stream
 #pragma unroll
         for (int u = 0; u < PX_U; u++) {
-            if (!ok[u] || jj[u] == k) continue;
-            const int res = prefilter_q<DIMS>(R, k, pk, sk, jj[u], q[u]);
-            if (res & 2) {
-                if (cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = jj[u];
-                cnt++;
-            }
+            int res = (ok[u] && jj[u] != k) ? prefilter_sel<DIMS>(R, pk, sk, q[u]) : 0;
+            if (res & 4) res = exact_pair<DIMS>(R, k, jj[u]);
+            const int acc = (res >> 1) & 1;
+            if (acc && cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = jj[u];
+            cnt += acc;
         }
     }
 }
