@@ -320,11 +320,10 @@ def bench_large(args, nat, synth):
     return out
 
 
-def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300, 0.005, 0.02), retune=False,
-                chunk=8):
+def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300, 0.005, 0.02), chunk=8):
     """Admit `specs` (cell divisor cd, build launch shape) into `streams`
-    batches, relax, warm up, optionally re-tune the launch shape on the full
-    batch, then time `steps` steps.  Returns the measurements."""
+    batches, relax, warm up, then time `steps` steps.  Returns the
+    measurements."""
     if streams > 1:
         b = nat.BatchGroup(len(specs) + 1, streams=streams, graph=graph)
     else:
@@ -341,11 +340,6 @@ def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300
     t_admit = time.perf_counter() - t0
     b.minimize(*relax)
     b.step(warmup)
-    full_tune = None
-    if retune:
-        tab, best = b.autotune(100)
-        shape = (tab[best]["block_size"], tab[best]["lanes_per_particle"])
-        full_tune = [dict(e, ms_per_step=round(e["ms_per_step"], 5)) for e in tab]
     reb0 = sum(b.stats(i)["rebuilds"] for i in ids)
     b.step(steps)
     ms = b.info()["last_ms"]
@@ -361,7 +355,6 @@ def run_batched(nat, specs, cd, shape, streams, graph, warmup, steps, relax=(300
     b.close() if hasattr(b, "close") else None
     return {"value": tot * steps / (ms / 1e3), "ms_per_step": ms / steps, "replicas": len(ids), "particles": tot,
             "rebuilds_per_step": reb / steps, "admit_s": round(t_admit, 2), "shape": shape,
-            "autotune_full_batch": full_tune,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                          "kernel": "whole batched step (all kernels)",
@@ -461,14 +454,11 @@ def bench_batched(args, nat):
         shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else (256, 4)
         cd, tune = args.cell_divisor, None
     else:
-        cd, shape, tune = W.autotune_batch(specs, pilot=64, steps=60)
+        cd, shape, tune = W.autotune_group(specs, streams=args.streams, chunk=args.chunk)
     log(f"[bench] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
         f"({time.perf_counter() - t0:.1f}s)")
-    # the launch shape changes no result: with the autotuner on, it is re-tuned
-    # on the full batch (the pilot holds a quarter of the replicas, so fewer
-    # rebuild per step)
     res = run_batched(nat, specs, cd, shape, args.streams, args.graph, args.warmup, args.steps,
-                      retune=not args.cell_divisor, chunk=args.chunk)
+                      chunk=args.chunk)
     log(f"[bench] batched: {res['ms_per_step'] * 1e3:.1f} us/step over {res['particles']} particles")
     return {"metric": METRIC, "value": res["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -478,8 +468,7 @@ def bench_batched(args, nat):
                                    "75% 3D / 25% 2D, Brownian dt=1e-4, kBT U[0.1,0.3], per-replica seeds",
                        "replicas": res["replicas"], "particles": res["particles"], "graph": args.graph,
                        "streams": args.streams, "cell_divisor": cd, "build_block": res["shape"][0],
-                       "build_lanes_per_slot": res["shape"][1], "autotune": tune,
-                       "autotune_full_batch": res["autotune_full_batch"]},
+                       "build_lanes_per_slot": res["shape"][1], "chunk": args.chunk, "autotune": tune},
             "rebuilds_per_step": res["rebuilds_per_step"], "roofline": res["roofline"], "cpu_baseline": None}
 
 

@@ -258,7 +258,7 @@ def bench(args, log):
     rng = np.random.default_rng(args.seed)
     pilot = [W.served_job(rng, k)[0] for k in range(24)]
     t0 = time.perf_counter()
-    cd, shape, _ = W.autotune_batch(pilot, pilot=24, steps=40)
+    cd, shape, _ = W.autotune_group(pilot, streams=1, chunk=8, steps=40)
     log(f"[serve] autotuner: cell divisor {cd}, build block {shape[0]} x {shape[1]} lanes/slot "
         f"({time.perf_counter() - t0:.1f}s)")
     sweep = []

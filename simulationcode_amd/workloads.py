@@ -84,37 +84,6 @@ def batched_specs(n_replicas=256, seed=20261207, n_lo=500, n_hi=20000, dt=1e-4):
     return out
 
 
-def autotune_batch(specs, cell_divisors=(1, 2, 3), pilot=64, steps=100, relax=(300, 0.005, 0.02)):
-    """Workload-level autotuner: cell size, then launch shape.
-
-    For each candidate cell divisor (cells of side >= R_v / k, stencil +-k)
-    a pilot batch of the first `pilot` replicas of `specs` is relaxed and
-    stepped, the library's launch-shape autotuner (block size x lanes per
-    slot, sc_batch_autotune) runs on it, and the best shape's time per
-    particle-step is recorded.  Returns (best cell divisor, best shape
-    (block_size, lanes_per_particle), table).  The cell divisor changes the
-    slot order and so the bits of a replica's trajectory: it is applied to
-    every replica of a run (alone or in a mix) alike.  The launch shape
-    does not change any result."""
-    sub = specs[:pilot]
-    table = []
-    for cd in cell_divisors:
-        b = nat.Batch(len(sub) + 1)
-        npart = 0
-        for s in sub:
-            p, D, r, v = replica(**dict(s, cell_divisor=cd))
-            b.add(p, D, r, v)
-            npart += s["N"]
-        b.minimize(*relax)
-        b.step(20)
-        shapes, best = b.autotune(steps)
-        b.close()
-        for e in shapes:
-            table.append(dict(e, cell_divisor=cd, ns_per_particle_step=e["ms_per_step"] * 1e6 / npart))
-    best = min(table, key=lambda e: e["ms_per_step"])
-    return best["cell_divisor"], (best["block_size"], best["lanes_per_particle"]), table
-
-
 def served_job(rng, job_id, n_lo=1000, n_hi=200000, steps_lo=1000, steps_hi=100000):
     """One served job: replica() keywords plus its step count."""
     dims = 2 if rng.uniform() < 0.5 else 3
@@ -126,3 +95,41 @@ def served_job(rng, job_id, n_lo=1000, n_hi=200000, steps_lo=1000, steps_hi=1000
     return dict(N=log_uniform_int(rng, n_lo, n_hi), dims=dims, system=system, kBT=float(rng.uniform(0.1, 0.3)),
                 dt=dt, gamma=1.0, seed=int(rng.integers(1, 2 ** 62)), precision="mixed"), \
         log_uniform_int(rng, steps_lo, steps_hi)
+
+
+# build launch shapes tried by autotune_group: one lane per slot (the radius
+# form, the R_v sweep) and the R_v sweep with several lanes per slot
+GROUP_SHAPES = ((256, 0), (256, 1), (256, 2), (256, 4), (256, 8), (256, 16), (64, 4), (64, 8))
+
+
+def autotune_group(specs, cell_divisors=(1, 2, 3), shapes=GROUP_SHAPES, streams=4, chunk=12, steps=120,
+                   relax=(300, 0.005, 0.02), warmup=20):
+    """Workload-level autotuner that measures every choice as it runs: for
+    each cell divisor, all replicas of `specs` are admitted into `streams`
+    concurrently stepped batches (steps launched ahead in chunks of `chunk`),
+    relaxed and warmed up; then each build launch shape steps the whole group
+    `steps` steps and its wall time per step is recorded.  The state advances
+    through the measurements (every shape writes the same lists, so the
+    trajectory does not depend on the order).  Returns (best cell divisor,
+    best shape, table).  The cell divisor sets the slot order and so the bits
+    of a trajectory: it is applied to every replica of a run alike."""
+    table = []
+    for cd in cell_divisors:
+        b = nat.BatchGroup(len(specs) + 1, streams=streams, graph=False) if streams > 1 \
+            else nat.Batch(len(specs) + 1, graph=False)
+        b.set_ahead(chunk)
+        for spec in specs:
+            p, D, r, v = replica(**dict(spec, cell_divisor=cd))
+            b.add(p, D, r, v)
+        b.minimize(*relax)
+        b.step(warmup)
+        for shape in shapes:
+            b.set_build_launch(*shape)
+            b.step(steps)
+            ms = b.info()["last_ms"]
+            table.append({"cell_divisor": cd, "block_size": shape[0], "lanes_per_particle": shape[1],
+                          "ms_per_step": ms / steps})
+        b.close()
+    best = min(table, key=lambda e: e["ms_per_step"])
+    return best["cell_divisor"], (best["block_size"], best["lanes_per_particle"]), table
+

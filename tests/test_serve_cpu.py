@@ -136,3 +136,28 @@ def test_failed_job_is_retired_and_the_others_complete(monkeypatch):
     assert by_id[1].error == serve.nat.SC_ERR_NONFINITE
     assert by_id[0].error is None and by_id[2].error is None
     assert srv.used_bytes == 0 and not srv.running
+
+
+def test_workload_generators_follow_the_stated_ranges():
+    """The job and replica generators of the served and batched workloads
+    (no device work): N, dimensions, dynamics, temperatures and step counts
+    inside the stated ranges, and seeded (the same stream twice)."""
+    import numpy as np
+
+    from simulationcode_amd import workloads as W
+
+    jobs = [W.served_job(np.random.default_rng(5), k) for k in range(1)]
+    rng = np.random.default_rng(11)
+    many = [W.served_job(rng, k) for k in range(400)]
+    assert [j[0]["N"] for j in jobs] == [W.served_job(np.random.default_rng(5), 0)[0]["N"]]
+    for spec, steps in many:
+        assert 1000 <= spec["N"] <= 200000 and spec["dims"] in (2, 3)
+        assert spec["system"] in ("newtonian", "langevin", "brownian")
+        assert 0.1 <= spec["kBT"] <= 0.3 and 1000 <= steps <= 100000
+    assert {s["system"] for s, _ in many} == {"newtonian", "langevin", "brownian"}
+    specs = W.batched_specs(256)
+    assert len(specs) == 256 and specs == W.batched_specs(256)
+    assert all(500 <= s["N"] <= 20000 and s["system"] == "brownian" for s in specs)
+    frac3 = np.mean([s["dims"] == 3 for s in specs])
+    assert 0.6 < frac3 < 0.9
+    assert callable(W.autotune_group) and W.GROUP_SHAPES[0] == (256, 0)
