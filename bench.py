@@ -25,6 +25,9 @@ r (24) + D (8) + v read/write (48) + F write (24) + 4 (n_half+1) neighbour
 index bytes, n_half measured on the device lists; its average launch time
 comes from HIP events on the context stream in the timed-kernel run.
 
+small_configs: the two small stated systems (LJ N=1000 Newtonian; polydisperse
+N=8000 Langevin, FP64 and mixed), see small_configs.
+
 batched / served: bounded runs of the replica workloads (extra keys), see
 batched_summary and simulationcode_amd/serve.summary.
 
@@ -34,10 +37,12 @@ median of three runs, once on one thread and once on all host cores.
 
 --workload batched: 256 independent replicas (N log-uniform in [500, 20000],
 75 % 3D rho 1.02 / 25 % 2D rho 1.0, kBT in [0.1, 0.3], Brownian, per-replica
-seeds, mixed precision) split over four batches stepped concurrently on their
+seeds, mixed precision) split over three batches stepped concurrently on their
 own HIP streams, steps launched ahead in chunks of 12 (a captured hipGraph
-replay, one to three or six to eight streams and chunks of 4-16 steps were all
-measured slower); value = aggregate
+replay, one, two or five to eight streams and chunks of 4-16 steps were all
+measured slower; four streams were 2% faster only when no other stream,
+the default stream included, had been created in the process before them:
+with four hardware queues per process, five streams share one); value = aggregate
 particle-steps/s over the wall time of the timed steps.  dt = 1e-4: at the stated dt = 0.001 explicit
 Euler-Maruyama blows up within 10^2-10^3 steps for this potential.
 --workload served: see simulationcode_amd/serve.py.
@@ -307,6 +312,8 @@ def bench_large(args, nat, synth):
     else:
         ctx.close()
 
+    small, small_states = small_configs(args, nat)
+    out["small_configs"] = small
     if not args.no_batched:
         out["batched"] = batched_summary(args, nat)
     if not args.no_served:
@@ -315,6 +322,7 @@ def bench_large(args, nat, synth):
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 thread and all host cores)")
         out["cpu_baseline"] = cpu_baseline((state["r"], state["v"], state["F"], D, L, N))
+        out["cpu_baseline"]["small_configs"] = cpu_small_configs(small_states)
     else:
         out["cpu_baseline"] = None
     return out
@@ -421,14 +429,130 @@ def W_replica(spec, cd):
     return W.replica(**dict(spec, cell_divisor=cd))
 
 
+def tuned_shape(nat, p, D, r, v, steps=20):
+    """The build launch shape sc_autotune picks for a system, tuned on a
+    copy of it (the measured context then starts from the stated state)."""
+    ctx = nat.Context(p, D)
+    ctx.set_state(r=r, v=v)
+    ctx.update_neighbor_lists()
+    ctx.calculate_new_forces(None)
+    table, best = ctx.autotune(steps)
+    ctx.close()
+    return table[best]["block_size"], table[best]["lanes_per_particle"]
+
+
+def small_configs(args, nat):
+    """The two small stated systems on the device (extra key `small_configs`),
+    each from a seeded start, device time of the timed steps from HIP events:
+    LJ: 3D Lennard-Jones N=1000, rho=1 (box 10), simple-cubic lattice with
+    +-0.1 jitter, Newtonian velocity Verlet dt=0.01 from kBT=0.1, FP64, 1000
+    steps.  Polydisperse: 3D soft spheres N=8000, rho=1.02, Langevin
+    kBT=0.15, gamma=1, dt=0.01 (on-device noise) after 200 relaxation
+    iterations (FP64), 10^4 steps in FP64 and in the mixed form from that
+    state, with the mixed form's pair-force error there.  Both are
+    latency-bound (a few microseconds of launch and dependency per kernel):
+    the rate is what one small system gets alone.  The build launch shape of
+    each is the one sc_autotune picks on a copy of the system.  Returns the
+    results and the states the CPU baseline times on."""
+    from simulationcode_amd import workloads as W
+    out, states = {}, {}
+    p, D, r, v = W.replica(1000, 3, potential="lj", system="newtonian", kBT=0.1, dt=0.01, rho=1.0,
+                           precision="fp64", seed=args.seed + 1, cell_divisor=0)
+    shape = tuned_shape(nat, p, D, r, v)
+    ctx = nat.Context(p, D)
+    ctx.set_build_launch(*shape)
+    ctx.set_state(r=r, v=v)
+    ctx.update_neighbor_lists()
+    ctx.calculate_new_forces(None)
+    states["lj"] = (p, D, ctx.get_state())
+    r0 = ctx.rebuild_count()
+    ctx.do_time_steps(1000)
+    ms = ctx.last_elapsed_ms()
+    out["lj_N1000_newtonian_fp64"] = {"value": 1000 * 1000 / (ms / 1e3), "unit": "particle-steps/s",
+                                      "us_per_step": ms, "steps": 1000, "rebuilds": ctx.rebuild_count() - r0,
+                                      "build_shape": list(shape)}
+    ctx.close()
+    res = {}
+    start = None
+    for name, prec in (("fp64", nat.PREC_FP64), ("mixed", nat.PREC_MIXED)):
+        p, D, r, v = W.replica(8000, 3, potential="berthier", system="langevin", kBT=0.15, dt=0.01, rho=1.02,
+                               precision="mixed" if prec == nat.PREC_MIXED else "fp64", seed=args.seed + 2,
+                               cell_divisor=0)
+        ctx = nat.Context(p, D)
+        if start is None:  # relax the seeded lattice once (FP64); both forms start from that state
+            ctx.set_state(r=r, v=np.zeros_like(v))
+            ctx.update_neighbor_lists()
+            ctx.minimize(200, 0.005, 0.02)
+            start = ctx.get_state(F=False)["r"]
+            shape = tuned_shape(nat, p, D, start, v)
+        ctx.set_build_launch(*shape)
+        # pair forces on the common state: zero velocities and injected zero
+        # noise, so that the Langevin terms vanish in both forms
+        ctx.set_state(r=start, v=np.zeros_like(v))
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(np.zeros_like(v))
+        Fpair = ctx.get_state(r=False, v=False, F=True)["F"]
+        ctx.set_state(v=v)
+        ctx.update_neighbor_lists()
+        ctx.calculate_new_forces(None)
+        if name == "fp64":
+            states["poly"] = (p, D, ctx.get_state())
+        r0 = ctx.rebuild_count()
+        ctx.do_time_steps(10 ** 4)
+        ms = ctx.last_elapsed_ms()
+        res[name] = {"value": 8000 * 1e4 / (ms / 1e3), "unit": "particle-steps/s", "us_per_step": ms / 10.0,
+                     "steps": 10 ** 4, "rebuilds": ctx.rebuild_count() - r0, "build_shape": list(shape),
+                     "F": Fpair}
+        ctx.close()
+    F64, F32 = res["fp64"].pop("F"), res["mixed"].pop("F")
+    fn = np.linalg.norm(F64, axis=1)
+    res["mixed"].update(speedup_vs_fp64=res["mixed"]["value"] / res["fp64"]["value"],
+                        max_rel_force_err_vs_mean=float(np.linalg.norm(F32 - F64, axis=1).max() / fn.mean()))
+    out["poly_N8000_langevin"] = res
+    log(f"[bench] small configs: LJ N=1000 {out['lj_N1000_newtonian_fp64']['us_per_step']:.1f} us/step; "
+        f"polydisperse N=8000 Langevin FP64 {res['fp64']['us_per_step']:.1f} us/step, "
+        f"mixed {res['mixed']['us_per_step']:.1f} us/step")
+    return out, states
+
+
+def cpu_small_configs(states, budget_s=4.0):
+    """The oracle (serial FP64) on the same two small systems' starts: LJ
+    N=1000 Newtonian, and the polydisperse N=8000 Langevin step with injected
+    noise; steps sized to about budget_s / 2 each after 10 warm-up steps."""
+    import oracle
+    out = {}
+    for key, name in (("lj", "lj_N1000_newtonian_fp64"), ("poly", "poly_N8000_langevin")):
+        p, D, st = states[key]
+        lj = key == "lj"
+        op = oracle.make_params(dims=3, N=p.N, box_size=p.box_size, force_cutoff=p.force_cutoff,
+                                skin=p.skin_distance, D=D, potential=oracle.POT_LJ if lj else oracle.POT_BERTHIER,
+                                system=oracle.SYS_NEWTONIAN if lj else oracle.SYS_LANGEVIN, kBT=p.kBT,
+                                gamma=p.gamma, dt=p.dt)
+        ost = oracle.State(op, st["r"], st["v"], D, F=st["F"])
+        ost.update_neighbor_lists(cells=True)
+        rng = np.random.default_rng(7)
+        noise = None if lj else rng.standard_normal((10, p.N, 3))
+        t0 = time.perf_counter()
+        ost.do_time_steps(10, noise, cells=True)
+        per = (time.perf_counter() - t0) / 10
+        n = int(min(max(budget_s / 2 / max(per, 1e-6), 10), 2000))
+        noise = None if lj else rng.standard_normal((n, p.N, 3))
+        t0 = time.perf_counter()
+        ost.do_time_steps(n, noise, cells=True)
+        dt_s = time.perf_counter() - t0
+        out[name] = {"value": p.N * n / dt_s, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                     "sample": f"{n} steps after 10 warm-up steps" + ("" if lj else " (injected host noise)")}
+    return out
+
+
 # measured best launch choices of the 256-replica batch
-BATCHED_CD, BATCHED_SHAPE, BATCHED_STREAMS, BATCHED_CHUNK = 2, (256, 0), 4, 12
+BATCHED_CD, BATCHED_SHAPE, BATCHED_STREAMS, BATCHED_CHUNK = 2, (256, 0), 3, 12
 
 
 def batched_summary(args, nat, steps=300, warmup=20):
     """The batched workload at a bounded length for the default run: the 256
     replicas at the best measured launch choices (cell divisor 2, the
-    per-slot radius build {256, 0}, four streams, chunks of 12 steps)."""
+    per-slot radius build {256, 0}, three streams, chunks of 12 steps)."""
     from simulationcode_amd import workloads as W
     t0 = time.perf_counter()
     specs = W.batched_specs(args.replicas)
@@ -489,7 +613,7 @@ def main():
                     help="large: steps of each form's long window (at least five rebuild intervals)")
     ap.add_argument("--graph", action="store_true",
                     help="batched: replay the step as a captured hipGraph (eager launches measured faster)")
-    ap.add_argument("--streams", type=int, default=4,
+    ap.add_argument("--streams", type=int, default=3,
                     help="batched: split the replicas over this many batches on their own streams")
     ap.add_argument("--drift-steps", type=int, default=20000)
     ap.add_argument("--replicas", type=int, default=256)

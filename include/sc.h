@@ -295,7 +295,9 @@ int sc_batch_info(sc_batch *b, int32_t *active, int64_t *particles, int64_t *dev
  * lanes_per_particle lanes cooperating on one slot (1, 2, 4, 8, 16 or 32);
  * block_size 256 with lanes_per_particle 0 selects the per-slot radius
  * form (one lane per slot, searching only out to the slot's own largest
- * half-list radius; the default of a single system).
+ * half-list radius).  A new context starts with a shape chosen by its
+ * size: {64, 16} below 20000 particles, {256, 4} below 60000, {256, 1}
+ * below 200000, else {256, 0}.
  * The lists, and so every result, are identical for every shape; only the
  * time differs (one lane per slot suits one large system, several lanes
  * suit batches of small replicas whose rebuild is latency-bound). */

@@ -372,6 +372,11 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
     c->geom.any_tile_build = h.build_kind == BUILD_SLOT ? (h.bf_on ? 2 : 1) : 0;
     c->geom.any_rows_build = h.build_kind == BUILD_ROWS ? 1 << h.dims : 0;
     c->geom.flag = reinterpret_cast<const int *>(reinterpret_cast<const char *>(h.dyn) + offsetof(RepDyn, rebuild));
+    // default build launch shape by system size (the shapes sc_autotune picked
+    // for 3D polydisperse systems of 1000 to 300000 particles): several lanes
+    // per slot cut the latency of a small system's rebuild, one lane per slot
+    // with the per-slot radius suits a large one; the lists are the same
+    sc::default_build_shape(h.n, c->geom.build_bs, c->geom.build_lpp);
     rc = SC_OK;
     rc |= dalloc(c, &c->d_rep, 1);
     rc |= dalloc(c, &c->d_tiles, c->ntiles);

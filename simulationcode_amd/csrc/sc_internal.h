@@ -171,6 +171,23 @@ struct RebuildGeom {
                              //   per-slot radius form); the lists do not depend on it
 };
 bool build_shape_valid(int bs, int lpp);
+// default per-slot build launch shape of a single system of n particles
+inline void default_build_shape(int n, int &bs, int &lpp)
+{
+    if (n < 20000) {
+        bs = 64;
+        lpp = 16;
+    } else if (n < 60000) {
+        bs = 256;
+        lpp = 4;
+    } else if (n < 200000) {
+        bs = 256;
+        lpp = 1;
+    } else {
+        bs = 256;
+        lpp = 0;
+    }
+}
 // rebuild work list: header {tiles, replicas, window runs, 0}, then the
 // replicas, the tiles and the (replica, run) pairs of the rebuilding replicas
 inline int plan_ints(int nreps, int ntiles, int nwin) { return 4 + nreps + ntiles + 2 * nwin; }
