@@ -101,7 +101,7 @@ struct Rep {
     // pos_alt decides
     int bf_on;
     float bf_lo, bf_hi, bf_hlo, bf_hhi, bf_m, bf_rh2lo, bf_rh2hi;
-    float bf_S, bf_rcut, bf_skin, bf_zeta;  // FP32 2^32 / L, sqrt(force_cutoff2), skin, zeta
+    float bf_c1, bf_c2, bf_zeta;  // FP32 rcut S / 2, skin S (S = 2^32 / L), zeta
     double wcell;        // cell side L / nc
     double L, invL, cinv;          // box, 1/box, cells per unit length
     double fc2, rcut, skin, Rv2, thr;

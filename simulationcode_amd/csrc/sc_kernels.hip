@@ -1384,6 +1384,18 @@ __device__ __forceinline__ int exact_pair(const Rep &R, int k, int j)
     return d2 < rh * rh ? 3 : 1;
 }
 
+// Certain-pass / certain-fail bounds on the squared half-list radius of the
+// Berthier pair (sk, sj) in fixed-point units, from the FP32 diameters (the
+// evaluation the prefilter margins are set for).
+__device__ __forceinline__ void prefilter_half_bounds(const Rep &R, float sk, float sj, float &lo, float &hi)
+{
+    const float t = fmaf(-R.bf_zeta, fabsf(sk - sj), 1.0f);
+    const float rh = fmaf(R.bf_c1, (sk + sj) * t, R.bf_c2);  // (rcut sigma_ij + skin) S
+    const float rh2 = rh * rh;
+    lo = fmaf(rh2, R.bf_hlo, -R.bf_m);
+    hi = fmaf(rh2, R.bf_hhi, R.bf_m);
+}
+
 // Fixed-point prefilter (R.bf_on): the pair's separation from the wrapping
 // differences of the rebuild-time fixed-point coordinates (the minimum image
 // without a shift), squared and summed in FP32, and the half-list radius in
@@ -1405,14 +1417,7 @@ __device__ __forceinline__ int prefilter_q(const Rep &R, int k, const uint4 &pk,
     bool exact = d2 >= R.bf_lo;
     if (!exact) {
         float lo = R.bf_rh2lo, hi = R.bf_rh2hi;
-        if (R.pot == 2) {
-            const float sj = __uint_as_float(q.w);
-            const float md = (sk + sj) * 0.5f * (1.0f - R.bf_zeta * fabsf(sk - sj));
-            const float rh = (R.bf_rcut * md + R.bf_skin) * R.bf_S;
-            const float rh2 = rh * rh;
-            lo = rh2 * R.bf_hlo - R.bf_m;
-            hi = rh2 * R.bf_hhi + R.bf_m;
-        }
+        if (R.pot == 2) prefilter_half_bounds(R, sk, __uint_as_float(q.w), lo, hi);
         if (d2 < lo)
             res = 3;
         else
@@ -1568,6 +1573,17 @@ __device__ inline float cell_gap32(int o, float f)
     return g > 1e-4f ? g - 1e-4f : 0.0f;
 }
 
+// Square root for the x extent of a stencil row: the hardware estimate
+// (within 1 ulp, below 1e-6 for the arguments here, which are at most
+// (s + 1)^2 in cell units) instead of the correctly rounded sequence, which
+// costs ~15 more instructions.  Every caller widens the extent by 1e-4 cell
+// units, so the cells it lists still cover the radius, and the list entries
+// are decided by the exact predicates of the cells' particles alone.
+__device__ __forceinline__ float sqrt_extent(float x)
+{
+    return __builtin_amdgcn_sqrtf(x);
+}
+
 template <int DIMS, class Seg>
 __device__ __forceinline__ void for_stencil_ranges(const Rep &R, int c0, int c1, int c2, float fx, float fy, float fz, Seg &&seg)
 {
@@ -1581,7 +1597,7 @@ __device__ __forceinline__ void for_stencil_ranges(const Rep &R, int c0, int c1,
             const float gy = cell_gap32(oy, fy);
             const float g2 = gy * gy + gz * gz;
             if (g2 > rc2) continue;
-            const float rx = sqrtf(rc2 - g2) + 1e-4f;
+            const float rx = sqrt_extent(rc2 - g2) + 1e-4f;
             const int lo = max(-s, (int)floorf(fx - rx)), hi = min(s, (int)floorf(fx + rx));
             if (lo > hi) continue;
             const int row = nc * (wrapc(c1 + oy, nc) + nc * cz);
@@ -1885,7 +1901,7 @@ __device__ __forceinline__ void px_run(float r2, float g2, float fx, int s, int 
     lo = 1;
     hi = 0;
     if (g2 > r2) return;
-    const float rx = sqrtf(r2 - g2) + 1e-4f;
+    const float rx = sqrt_extent(r2 - g2) + 1e-4f;
     lo = max(-s, (int)floorf(fx - rx));
     hi = min(s, (int)floorf(fx + rx));
 }
@@ -1905,14 +1921,7 @@ __device__ __forceinline__ int prefilter_sel(const Rep &R, const uint4 &pk, floa
         d2 = d2 + fz * fz;
     }
     float lo = R.bf_rh2lo, hi = R.bf_rh2hi;
-    if (R.pot == 2) {
-        const float sj = __uint_as_float(q.w);
-        const float md = (sk + sj) * 0.5f * (1.0f - R.bf_zeta * fabsf(sk - sj));
-        const float rh = (R.bf_rcut * md + R.bf_skin) * R.bf_S;
-        const float rh2 = rh * rh;
-        lo = rh2 * R.bf_hlo - R.bf_m;
-        hi = rh2 * R.bf_hhi + R.bf_m;
-    }
+    if (R.pot == 2) prefilter_half_bounds(R, sk, __uint_as_float(q.w), lo, hi);
     const bool hin = d2 < lo;
     const bool band = d2 >= R.bf_lo || (!hin && d2 <= hi);
     return d2 > R.bf_hi ? 0 : (band ? 4 : (hin ? 3 : 1));

@@ -84,11 +84,17 @@ unsigned diameter_fixed(double D, double dqinv) { return (unsigned)(long long)st
 //   |d2_32 - D2| <= 6 2^-24 D2 + 2.1 sqrt(dims) sqrt(D2) + 1.1 dims =: M(D2),
 // increasing in D2, and the FP64 d2 of the same positions differs from D2 by
 // far less.  Four times M(R_v^2 S^2) is taken on both sides of R_v^2 S^2.
-// The half-list radius rcut sigma_ij + skin evaluated in FP32 from FP32
-// diameters, times S and squared, is off by at most (23 + 16 zeta max D)
-// 2^-24 relative while 1 - zeta |D_i - D_j| >= 1/2 (the subtraction then
-// cannot amplify the error more than twofold); eight times that is the
-// relative margin, plus the absolute 4 M(R_v^2 S^2).  The LJ radius is one
+// The half-list radius (rcut sigma_ij + skin) S is evaluated in FP32 from
+// FP32 diameters as fma(c1, (D_i + D_j) t, c2), t = fma(-zeta, |D_i - D_j|, 1),
+// with the constants c1 = rcut S / 2 and c2 = skin S rounded once each, and
+// then squared.  In units of u = 2^-24, while t >= 1/2: |D_i - D_j| is off
+// by at most 2 u max D (diameter roundings and the subtraction), so t by
+// (6 zeta max D + 1) u relative (zeta's rounding, the fused rounding, and
+// the factor 2 from t >= 1/2); D_i + D_j by 2 u, their product by
+// (4 + 6 zeta max D) u, the radius by (6 + 6 zeta max D) u (c1's rounding
+// and the fused rounding; the skin term's error is smaller) and its square
+// by (13 + 12 zeta max D) u.  The margin keeps eight times the larger
+// (23 + 16 zeta max D) u, plus the absolute 4 M(R_v^2 S^2).  The LJ radius is one
 // FP64 constant, given 2^-20 relative slack and the same absolute margin.
 void build_prefilter_setup(Rep &h)
 {
@@ -98,9 +104,8 @@ void build_prefilter_setup(Rep &h)
     h.bf_lo = (float)(RvS2 - m4);
     h.bf_hi = (float)(RvS2 + m4);
     h.bf_m = (float)m4;
-    h.bf_S = (float)S;
-    h.bf_rcut = (float)h.rcut;
-    h.bf_skin = (float)h.skin;
+    h.bf_c1 = (float)(0.5 * h.rcut * S);
+    h.bf_c2 = (float)(h.skin * S);
     h.bf_zeta = (float)h.zeta;
     const double dmax = h.dmax, dmin = h.dmin;
     const double eh = 8.0 * (23.0 + 16.0 * std::fabs(h.zeta) * dmax) * u;

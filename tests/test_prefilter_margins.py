@@ -1,15 +1,18 @@
 """The neighbour-list build's fixed-point FP32 prefilter, restated in numpy
-float32 (IEEE round-to-nearest, no contraction: the device code is compiled
-with -ffp-contract=off and forms every product and sum separately), checked
-against the FP64 predicates of the reference on adversarial pairs.
+float32 (IEEE round-to-nearest; the device code is compiled with
+-ffp-contract=off, so every product and sum is rounded separately except the
+explicit fused multiply-adds, restated here by fma32), checked against the
+FP64 predicates of the reference on adversarial pairs.
 
-The device test (simulationcode_amd/csrc/sc_kernels.hip, prefilter_q) is:
+The device test (simulationcode_amd/csrc/sc_kernels.hip, prefilter_q and
+prefilter_half_bounds) is:
   u = rint(x * 2^32 / L) mod 2^32 per coordinate (FP64 product, then rint);
   d2 = sum of squares of float32(int32(u_i - u_j)), in FP32, left to right;
   full list: d2 > hi -> out, d2 < lo -> in, else the FP64 test;
-  half list (Berthier): rh = (rcut * md + skin) * S in FP32 from FP32
-  diameters, md = (Di + Dj) * 0.5 * (1 - zeta |Di - Dj|);
-  in if d2 < rh^2 * hlo - m, out if d2 > rh^2 * hhi + m, else the FP64 test.
+  half list (Berthier): t = fma(-zeta, |Di - Dj|, 1), rh = fma(c1, (Di + Dj) t,
+  c2) in FP32 from FP32 diameters, with c1 = rcut S / 2 and c2 = skin S;
+  in if d2 < fma(rh^2, hlo, -m), out if d2 > fma(rh^2, hhi, m), else the
+  FP64 test.
 The margins (lo, hi, hlo, hhi, m) follow build_prefilter_setup
 (simulationcode_amd/csrc/sc_replica.hip).  The test places pairs at the
 FP64 thresholds and at relative offsets of 1e-9 .. 1e-3 around them, in
@@ -23,6 +26,28 @@ import numpy as np
 import pytest
 
 
+def fma32(a, b, c):
+    """Correctly rounded float32 a*b + c.  The float64 product of two
+    float32 values is exact; the float64 sum s carries the rounding error e
+    (TwoSum, p + c == s + e exactly), and rounding s to float32 differs from
+    rounding p + c only when s is a float32 midpoint and e != 0, where the
+    result is the neighbour on e's side."""
+    a, b, c = (np.asarray(x, dtype=np.float32).astype(np.float64) for x in (a, b, c))
+    p = a * b
+    s = p + c
+    bb = s - p
+    e = (p - (s - bb)) + (c - bb)
+    r = s.astype(np.float32)
+    r64 = r.astype(np.float64)
+    lo_n = np.nextafter(r, np.float32(-np.inf)).astype(np.float64)
+    hi_n = np.nextafter(r, np.float32(np.inf)).astype(np.float64)
+    mid = ((s - r64) * 2.0 == (hi_n - r64)) | ((r64 - s) * 2.0 == (r64 - lo_n))
+    above = np.where(r64 > s, r, np.nextafter(r, np.float32(np.inf)))
+    below = np.where(r64 < s, r, np.nextafter(r, np.float32(-np.inf)))
+    r = np.where(mid & (e > 0.0), above, np.where(mid & (e < 0.0), below, r))
+    return r.astype(np.float32)
+
+
 def margins(L, Rv, dims, zeta, dmax, rcut, skin, lj_md=None, shrink=1.0):
     S = 2.0 ** 32 / L
     u = 2.0 ** -24
@@ -30,8 +55,8 @@ def margins(L, Rv, dims, zeta, dmax, rcut, skin, lj_md=None, shrink=1.0):
     m4 = shrink * 4.0 * (6.0 * u * RvS2 + 2.1 * np.sqrt(dims) * Rv * S + 1.1 * dims)
     eh = shrink * 8.0 * (23.0 + 16.0 * abs(zeta) * dmax) * u
     out = dict(S=np.float32(S), lo=np.float32(RvS2 - m4), hi=np.float32(RvS2 + m4), m=np.float32(m4),
-               hlo=np.float32(1.0 - eh), hhi=np.float32(1.0 + eh), rcut=np.float32(rcut), skin=np.float32(skin),
-               zeta=np.float32(zeta))
+               hlo=np.float32(1.0 - eh), hhi=np.float32(1.0 + eh), c1=np.float32(0.5 * rcut * S),
+               c2=np.float32(skin * S), zeta=np.float32(zeta))
     if lj_md is not None:
         rh = rcut * lj_md + skin
         rh2 = rh * rh * S * S
@@ -56,10 +81,10 @@ def prefilter(ri, rj, Di, Dj, L, M, berthier=True):
     exact = (d2 >= M["lo"]) & (d2 <= M["hi"])
     if berthier:
         si, sj = Di.astype(np.float32), Dj.astype(np.float32)
-        md = (si + sj) * np.float32(0.5) * (np.float32(1.0) - M["zeta"] * np.abs(si - sj))
-        rh = (M["rcut"] * md + M["skin"]) * M["S"]
+        t = fma32(-M["zeta"], np.abs(si - sj), np.float32(1.0))
+        rh = fma32(M["c1"], (si + sj) * t, M["c2"])
         rh2 = rh * rh
-        lo, hi = rh2 * M["hlo"] - M["m"], rh2 * M["hhi"] + M["m"]
+        lo, hi = fma32(rh2, M["hlo"], -M["m"]), fma32(rh2, M["hhi"], M["m"])
     else:
         lo, hi = M["rh2lo"], M["rh2hi"]
     half = full & (d2 < lo)
@@ -146,3 +171,36 @@ def test_planted_small_margin_is_caught():
     rf, rhf = reference(ri, rj, Di, Dj, L, Rv, zeta, 1.25, 0.3)
     decided = ~exact
     assert not (np.array_equal(full[decided], rf[decided]) and np.array_equal(half[decided], rhf[decided]))
+
+
+def test_fma32_is_correctly_rounded():
+    """fma32 against exact rational arithmetic on random float32 triples, on
+    products that cancel the addend, and on sums whose float64 rounding lands
+    on a float32 midpoint (x + a b with a b = +-2^-24 (1 - 2^-36): the float64
+    sum is x +- 2^-24, exactly halfway, while the exact value lies inside)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(11)
+    n = 3000
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    c = rng.standard_normal(n).astype(np.float32)
+    k = slice(1000, 2000)  # near-cancellation of the product and the addend
+    c[k] = (-(a[k].astype(np.float64) * b[k]) * (1.0 + rng.uniform(-1e-6, 1e-6, 1000))).astype(np.float32)
+    m = slice(0, 1000)  # midpoints
+    x = rng.uniform(1.0, 2.0, 1000).astype(np.float32)
+    a[m] = np.float32(2.0 ** -24 * (1.0 + 2.0 ** -18))
+    b[m] = np.where(rng.uniform(size=1000) < 0.5, 1.0, -1.0).astype(np.float32) * np.float32(1.0 - 2.0 ** -18)
+    c[m] = x
+    s64 = a[m].astype(np.float64) * b[m] + c[m]
+    half = np.abs(s64 - x.astype(np.float64))
+    assert np.all(half == 2.0 ** -24)  # the float64 sum sits exactly halfway
+    got = fma32(a, b, c)
+    for i in range(n):
+        exact = Fraction(float(a[i])) * Fraction(float(b[i])) + Fraction(float(c[i]))
+        f = np.float32(float(exact))
+        cands = [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+        best = min(cands, key=lambda y: (abs(Fraction(float(y)) - exact), int(np.float32(y).view(np.uint32)) & 1))
+        assert got[i] == best, (i, a[i], b[i], c[i])
+    # without the midpoint correction, plain float32(float64 sum) is wrong on about half of them
+    plain = s64.astype(np.float32)
+    assert np.mean(plain != got[m]) > 0.2
