@@ -312,8 +312,10 @@ def bench_large(args, nat, synth):
     else:
         ctx.close()
 
-    small, small_states = small_configs(args, nat)
-    out["small_configs"] = small
+    small_states = None
+    if not args.no_small:
+        small, small_states = small_configs(args, nat)
+        out["small_configs"] = small
     if not args.no_batched:
         out["batched"] = batched_summary(args, nat)
     if not args.no_served:
@@ -322,7 +324,8 @@ def bench_large(args, nat, synth):
     if not args.no_cpu:
         log("[bench] cpu baseline (oracle, 1 thread and all host cores)")
         out["cpu_baseline"] = cpu_baseline((state["r"], state["v"], state["F"], D, L, N))
-        out["cpu_baseline"]["small_configs"] = cpu_small_configs(small_states)
+        if small_states is not None:
+            out["cpu_baseline"]["small_configs"] = cpu_small_configs(small_states)
     else:
         out["cpu_baseline"] = None
     return out
@@ -609,6 +612,7 @@ def main():
     ap.add_argument("--no-copy-ceiling", action="store_true", help="large: skip the measured device-copy ceiling")
     ap.add_argument("--no-batched", action="store_true", help="large: skip the bounded batched summary")
     ap.add_argument("--no-served", action="store_true", help="large: skip the bounded served summary")
+    ap.add_argument("--no-small", action="store_true", help="large: skip the stated small systems")
     ap.add_argument("--long-steps", type=int, default=1000,
                     help="large: steps of each form's long window (at least five rebuild intervals)")
     ap.add_argument("--graph", action="store_true",
