@@ -15,7 +15,8 @@
 //   nn     int[n]       short-list length of slot i
 //   nbr    int[cap*n]   short list, entry k of slot i at [k*n + i] so that a
 //                       wavefront of consecutive slots reads consecutive words
-//   cell_start int[ncell+1]  slot range of each cell (cell-sorted order)
+//   cell_start int[ncell+1]  slot range of each cell (cell-sorted order),
+//                       cell_start[ncell] = n
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -930,11 +930,11 @@ __device__ inline int rebuild_rep(const int *plan, const int *flag, int idx)
     return (idx == 0 && *gp(flag) != 0) ? 0 : -1;
 }
 
-// Slot range start of cell c (the scan leaves absolute starts); c == ncell
-// gives n.
+// Slot range start of cell c, 0 <= c <= ncell (the scan leaves absolute
+// starts and the sentinel cell_start[ncell] = n).
 __device__ inline int cstart(const Rep &R, int c)
 {
-    return c >= R.ncell ? R.n : gp(R.cell_start)[c];
+    return gp(R.cell_start)[min(c, R.ncell)];
 }
 
 // XCD-aware grid-stride iteration for capped grids: the blocks resident on
@@ -1067,6 +1067,7 @@ __global__ __launch_bounds__(SC_TB) void k_scan_local(const Rep *reps, const int
         run += v[q];
     }
     if (!fused && tid == SC_TB - 1) gp(R.chunk_off)[blockIdx.x] = sums[SC_TB - 1];  // chunk total, scanned by R2b
+    if (tid == 0 && c0 + SC_CHUNK >= R.ncell) gp(R.cell_start)[R.ncell] = R.n;  // end of the last cell
 }
 
 // R2c (after R2b, unfused scans only): chunk offsets added to the
