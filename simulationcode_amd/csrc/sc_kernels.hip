@@ -1940,6 +1940,7 @@ __device__ __forceinline__ void px_sweep(const Rep &R, int k, const uint4 &pk, f
         j = t.x;
         e = t.y;
     }
+    SC_AS1 int *out = gp(R.nbr) + (size_t)cnt * R.n + k;  // entry cnt of slot k; advanced by one row per store
     while (__ballot(j < e) != 0ull) {
         int jj[PX_U];
         bool ok[PX_U];
@@ -1963,7 +1964,10 @@ __device__ __forceinline__ void px_sweep(const Rep &R, int k, const uint4 &pk, f
             int res = (ok[u] && jj[u] != k) ? prefilter_sel<DIMS>(R, pk, sk, q[u]) : 0;
             if (res & 4) res = exact_pair<DIMS>(R, k, jj[u]);
             const int acc = (res >> 1) & 1;
-            if (acc && cnt < R.cap) gp(R.nbr)[(size_t)cnt * R.n + k] = jj[u];
+            if (acc && cnt < R.cap) {
+                *out = jj[u];
+                out += R.n;
+            }
             cnt += acc;
         }
     }
@@ -2058,7 +2062,9 @@ __device__ void build_tile_px(const Rep &R, const Tile &t, PxLds &S)
     gp(R.nn)[k] = cnt < R.cap ? cnt : R.cap;
 }
 
-__global__ __launch_bounds__(SC_TB) void k_build_px(const Tile *tiles, const Rep *reps, const int *plan,
+// Held to six wavefronts per SIMD (80 VGPRs, no spills): the store pointer
+// of px_sweep would otherwise take it to 81 VGPRs and five wavefronts.
+__global__ __launch_bounds__(SC_TB) __attribute__((amdgpu_waves_per_eu(6))) void k_build_px(const Tile *tiles, const Rep *reps, const int *plan,
                                                      const int *flag, int nreps, int ntiles)
 {
     __shared__ PxLds S;
