@@ -315,6 +315,28 @@ def test_prefilter_build_equals_fp64_sweep(kind, dims, N, cell_divisor, shape):
     assert out[0][3] == out[1][3] >= 2
 
 
+@pytest.mark.parametrize("dims,N,whole_chunks", [(3, 7200, True), (3, 5000, False), (2, 1430, True),
+                                                  (2, 1200, False)])
+def test_cell_offsets_end_at_n(dims, N, whole_chunks):
+    """The scan block holding the last cell also stores cell_start[ncell] = n,
+    which the builds read as the end of the last cell's slot range.  With the
+    cell count a whole number of 1024-cell scan chunks (16^3 and 32^2 cells
+    at cell divisor 2) and with a partial last chunk, the per-slot radius
+    build, the R_v sweep and the FP64 sweep give the oracle's pair sets."""
+    op, r, D, v, rc = H.make_system("berthier", N, dims, seed=131, relax=False)
+    nc = int(np.floor(2 * op.box_size / (op.verlet_sphere_size * (1.0 + 1e-10))))
+    assert ((nc ** dims) % 1024 == 0) == whole_chunks
+    lst = oracle.build_lists(op, r, D, cells=False)
+    for mode, shape in ((0, (256, 0)), (0, (256, 1)), (2, None)):
+        ctx = make_ctx(op, rc, r, v, D, cell_divisor=2, build_mode=mode, shape=shape)
+        ctx.update_neighbor_lists()
+        nf, full, nh, half = ctx.get_neighbor_lists()
+        ctx.close()
+        assert np.array_equal(nf, lst.nfull), (mode, shape)
+        assert np.array_equal(oracle._pairs(nf, full - 1, both=True), lst.full_pairs()), (mode, shape)
+        assert np.array_equal(oracle._pairs(nh, half - 1, both=False), lst.half_pairs()), (mode, shape)
+
+
 def test_prefilter_follows_the_diameter_range():
     """Diameters set past zeta (max D - min D) = 1/2, where the prefilter's
     FP32 half-list margin would not hold, return the automatic build to the
