@@ -626,15 +626,25 @@ __device__ void force_tile(const Rep &R, const Tile &t)
                     if (DIMS == 3) f[2] += fs * dz;
                 }
             }
-            for (; k < cnt; k++) {
-                const int j = nb[(size_t)k * n];
-                const double4 pj = ld4(gp(R.pos) + (j));
-                double dx, dy, dz;
-                double rij2 = sqdist<DIMS>(R, pi, pj, dx, dy, dz);
-                double fs = pair_fdivr64<DIMS, POT>(R, rij2, pi.w, pj.w);
-                f[0] += fs * dx;
-                f[1] += fs * dy;
-                if (DIMS == 3) f[2] += fs * dz;
+            if (k < cnt) {
+                // the last cnt - k < 4 entries, their loads issued together
+                // (index i stands in for the absent ones and is not used)
+                int jj[3];
+                double4 pp[3];
+#pragma unroll
+                for (int u = 0; u < 3; u++) jj[u] = k + u < cnt ? nb[(size_t)(k + u) * n] : i;
+#pragma unroll
+                for (int u = 0; u < 3; u++) pp[u] = ld4(gp(R.pos) + (jj[u]));
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    if (k + u >= cnt) break;
+                    double dx, dy, dz;
+                    double rij2 = sqdist<DIMS>(R, pi, pp[u], dx, dy, dz);
+                    double fs = pair_fdivr64<DIMS, POT>(R, rij2, pi.w, pp[u].w);
+                    f[0] += fs * dx;
+                    f[1] += fs * dy;
+                    if (DIMS == 3) f[2] += fs * dz;
+                }
             }
         } else {
             // mixed: 16-byte fixed-point records (one load per pair)
@@ -651,7 +661,19 @@ __device__ void force_tile(const Rep &R, const Tile &t)
 #pragma unroll
                 for (int u = 0; u < 4; u++) pair_mixed<DIMS, POT>(R, qi, Di, qq[u], f);
             }
-            for (; k < cnt; k++) pair_mixed<DIMS, POT>(R, qi, Di, ldq(R.spos + nb[(size_t)k * n]), f);
+            if (k < cnt) {
+                int jj[3];
+                uint4 qq[3];
+#pragma unroll
+                for (int u = 0; u < 3; u++) jj[u] = k + u < cnt ? nb[(size_t)(k + u) * n] : i;
+#pragma unroll
+                for (int u = 0; u < 3; u++) qq[u] = ldq(R.spos + jj[u]);
+#pragma unroll
+                for (int u = 0; u < 3; u++) {
+                    if (k + u >= cnt) break;
+                    pair_mixed<DIMS, POT>(R, qi, Di, qq[u], f);
+                }
+            }
         }
         // add_new_random_force! (src/MD.jl:30-43 Langevin, :67-78 Brownian)
         if (R.sys != 0) {
