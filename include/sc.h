@@ -77,7 +77,9 @@ typedef struct sc_ctx sc_ctx;
 
 /* initialize_neighbor_struct(skin, box_size, force_cutoff, N, D, dims;
  * maxneighbours) -- src/DataStructs.jl:349-378.  Allocates the device
- * state for one system; R_v = skin + force_cutoff * max(D). */
+ * state for one system; R_v = skin + force_cutoff * max(D).  Every
+ * diameter must be finite and positive (SC_ERR_ARG otherwise, before any
+ * device work; the same holds for the D of sc_set_state and sc_batch_add). */
 int sc_create(sc_ctx **out, const sc_params *p, const double *D);
 void sc_destroy(sc_ctx *ctx);
 const char *sc_last_error(const sc_ctx *ctx);

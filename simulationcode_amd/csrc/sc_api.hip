@@ -349,7 +349,7 @@ int sc_create(sc_ctx **out, const sc_params *p, const double *D)
 {
     if (!out || !p || !D) return SC_ERR_ARG;
     *out = nullptr;
-    if (!sc::params_valid(p)) return SC_ERR_ARG;
+    if (!sc::params_valid(p) || !sc::diameters_valid(D, p->N)) return SC_ERR_ARG;
     sc_ctx *c = new sc_ctx();
     c->p = *p;
     *out = c;
@@ -409,6 +409,7 @@ int sc_set_state(sc_ctx *c, const double *r, const double *v, const double *F, c
                  const double *disp)
 {
     if (!c || !c->d_rep) return SC_ERR_ARG;
+    if (D && !sc::diameters_valid(D, c->h.n)) return fail(c, SC_ERR_ARG, "diameters must be finite and positive");
     int rc;
     if (r && (rc = upload_field(c, r, FLD_R))) return rc;
     if (v && (rc = upload_field(c, v, FLD_V))) return rc;

@@ -308,7 +308,7 @@ const char *sc_batch_last_error(const sc_batch *b) { return b ? b->err.c_str() :
 
 int64_t sc_replica_device_bytes(const sc_params *p, const double *D)
 {
-    if (!p || !D || !sc::params_valid(p)) return -1;
+    if (!p || !D || !sc::params_valid(p) || !sc::diameters_valid(D, p->N)) return -1;
     return sc::replica_bytes(p, D);
 }
 
@@ -316,6 +316,7 @@ int sc_batch_add(sc_batch *b, const sc_params *p, const double *D, const double 
 {
     if (!b || !p || !D || !r || !id) return SC_ERR_ARG;
     if (!sc::params_valid(p)) return bfail(b, SC_ERR_ARG, "invalid parameters");
+    if (!sc::diameters_valid(D, p->N)) return bfail(b, SC_ERR_ARG, "diameters must be finite and positive");
     if (b->pending >= 0) {  // an asynchronous step's worker reads the slots: settle it first
         int rc = sc_batch_wait(b);
         if (rc) return rc;

@@ -244,6 +244,7 @@ int launch_min_mark(const Rep *reps, int nreps, bool begin, hipStream_t s);
 struct sc_params;
 namespace sc {
 bool params_valid(const sc_params *p);
+bool diameters_valid(const double *D, int64_t n);  // every D[i] finite and > 0
 int64_t replica_bytes(const sc_params *p, const double *D);
 int replica_init(Rep &h, const sc_params *p, const double *D, std::vector<void *> &allocs, std::string &err);
 // fixed-point diameter scale of the mixed form for diameters up to maxD, and

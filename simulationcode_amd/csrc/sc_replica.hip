@@ -25,6 +25,15 @@ bool params_valid(const sc_params *p)
              p->cell_divisor < 0 || p->cell_divisor > 4 || p->build_mode < 0 || p->build_mode > 4);
 }
 
+// Diameters set the Verlet radius, the cell grid and every pair's sigma_ij:
+// each must be a finite positive number (checked before any device work).
+bool diameters_valid(const double *D, int64_t n)
+{
+    for (int64_t i = 0; i < n; i++)
+        if (!(D[i] > 0.0) || !std::isfinite(D[i])) return false;
+    return true;
+}
+
 // Cell grid of a replica: cells of side >= R_v / k with a (2k+1)^dims
 // stencil; a hair of slack so floating-point cell assignment can never drop
 // a pair at exactly R_v.  Fewer than 2k+1 cells per side => one cell.

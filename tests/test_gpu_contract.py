@@ -14,6 +14,8 @@
 * The MD driver's log steps carry E_pot_test (src/MD.jl:623), and the
   minimiser's verbose report (src/InherentStructure.jl:57-69) leaves the
   descent unchanged.
+* Diameters that are not finite and positive are refused by sc_set_state
+  and sc_batch_add, and the state stays as it was.
 """
 import re
 
@@ -248,3 +250,28 @@ def test_minimiser_verbose_report_leaves_the_descent_unchanged(capsys):
     reports = [ln for ln in txt2.splitlines() if ln.startswith("(iteration, maxchange")]
     assert it > 1000 and len(reports) == 1 + (it - 1) // 1000
     assert reports[0].split("= (")[1].startswith("1, ")
+
+
+@pytest.mark.parametrize("bad", [np.nan, 0.0, -0.5])
+def test_bad_diameters_are_refused_and_the_state_kept(bad):
+    op, r, D, v, rc = H.make_system("berthier", 500, 3, seed=9)
+    ctx = _ctx(op, rc, r, v, D)
+    ctx.calculate_new_forces(None)
+    before = ctx.get_state()
+    D2 = D.copy()
+    D2[7] = bad
+    with pytest.raises(nat.ScError) as e:
+        ctx.set_state(r=r + 0.01, D=D2)
+    assert e.value.code == nat.SC_ERR_ARG
+    after = ctx.get_state()
+    for f in ("r", "v", "F"):
+        assert np.array_equal(before[f], after[f]), f
+    ctx.do_time_steps(20)  # still usable
+    assert np.all(np.isfinite(ctx.get_state()["r"]))
+    ctx.close()
+    b = nat.Batch(2)
+    with pytest.raises(nat.ScError) as e:
+        b.add(H.sc_params_from(op, rc), D2, r, v)
+    assert e.value.code == nat.SC_ERR_ARG
+    assert b.info()["active"] == 0
+    b.close()

@@ -43,6 +43,32 @@ def test_create_rejects_bad_arguments_without_touching_the_device():
     assert e.value.code == nat.SC_ERR_ARG
 
 
+def _valid_params(N=10):
+    p = nat.ScParams()
+    p.dims, p.N, p.potential, p.system, p.precision = 3, N, nat.POT_LJ, nat.SYS_NEWTONIAN, nat.PREC_FP64
+    p.maxneighbours, p.box_size, p.skin_distance = 50, 20.0, 0.3
+    p.force_cutoff, p.force_cutoff2 = 2.5, 6.25
+    return p
+
+
+def test_valid_params_pass_the_host_checks():
+    p = _valid_params()
+    assert nat.replica_device_bytes(p, np.ones(10)) > 0
+
+
+@pytest.mark.parametrize("bad", [np.nan, np.inf, 0.0, -1.0])
+def test_create_rejects_bad_diameters_without_touching_the_device(bad):
+    """A diameter that is not finite and positive is refused with SC_ERR_ARG
+    before any device call (sc_create, sc_replica_device_bytes)."""
+    p = _valid_params()
+    D = np.ones(10)
+    D[3] = bad
+    with pytest.raises(nat.ScError) as e:
+        nat.Context(p, D)
+    assert e.value.code == nat.SC_ERR_ARG
+    assert nat.replica_device_bytes(p, D) == -1
+
+
 # Random123 known-answer vectors for philox4x32-10 (Salmon et al., SC'11)
 KAT = [
     ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
