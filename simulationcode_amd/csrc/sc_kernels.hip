@@ -3171,10 +3171,19 @@ __global__ __launch_bounds__(SC_TB) void k_from_host(const Tile *tiles, const Re
     const double *st = R.stage;
     switch (field) {
     case FLD_R: {
+        // stored as the periodic image in [0, L), with the drift's wrap
+        // (src/MD.jl:520): the cell binning and the builds' image shifts
+        // take wrapped positions, and the reference's lists and forces
+        // depend on positions only modulo the box
         double4 p = ld4(gp(R.pos) + (k));
-        p.x = st[(size_t)o * d + 0];
-        p.y = st[(size_t)o * d + 1];
-        p.z = d == 3 ? st[(size_t)o * d + 2] : 0.0;
+        double x[3] = {0.0, 0.0, 0.0};
+        for (int c = 0; c < d; c++) {
+            x[c] = st[(size_t)o * d + c];
+            x[c] -= floor(x[c] / R.L) * R.L;
+        }
+        p.x = x[0];
+        p.y = x[1];
+        p.z = x[2];
         st4(gp(R.pos) + (k), p);
         put_shadow(R, k, p);
     } break;

@@ -16,6 +16,8 @@
   descent unchanged.
 * Diameters that are not finite and positive are refused by sc_set_state
   and sc_batch_add, and the state stays as it was.
+* Positions outside the box are their periodic images: the lists and forces
+  are those of the wrapped configuration.
 """
 import re
 
@@ -275,3 +277,35 @@ def test_bad_diameters_are_refused_and_the_state_kept(bad):
     assert e.value.code == nat.SC_ERR_ARG
     assert b.info()["active"] == 0
     b.close()
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_positions_outside_the_box_are_their_periodic_images(dims):
+    """The reference takes positions as given and works with them modulo the
+    box (minimum images in the force loops, src/MD.jl:107-133; CellListMap's
+    periodic cells in update_neighbor_lists!).  Moving a tenth of the
+    particles by +-L leaves the pair sets and the forces as they were."""
+    op, r, D, v, rc = H.make_system("berthier", 900 if dims == 2 else 1000, dims, seed=21)
+    L = op.box_size
+    rng = np.random.default_rng(3)
+    shifted = r.copy()
+    pick = rng.choice(len(r), len(r) // 10, replace=False)
+    shifted[pick] += rng.choice([-L, L], size=(len(pick), dims))
+    out = []
+    for pos in (r, shifted):
+        for mode, shape in ((0, (256, 0)), (0, (256, 1)), (2, None)):
+            ctx = nat.Context(H.sc_params_from(op, rc, cell_divisor=2, build_mode=mode), D)
+            if shape:
+                ctx.set_build_launch(*shape)
+            ctx.set_state(r=pos, v=v)
+            ctx.update_neighbor_lists()
+            ctx.calculate_new_forces(None)
+            nf, full, nh, half = ctx.get_neighbor_lists()
+            out.append((oracle._pairs(nf, full - 1, both=True), oracle._pairs(nh, half - 1, both=False),
+                        ctx.get_state()["F"]))
+            ctx.close()
+    lst = oracle.build_lists(op, r, D, cells=False)
+    for fp, hp, F in out:
+        assert np.array_equal(fp, lst.full_pairs())
+        assert np.array_equal(hp, lst.half_pairs())
+        assert H.force_rel_err(F, out[0][2]) <= 1e-12
