@@ -444,6 +444,11 @@ def tuned_shape(nat, p, D, r, v, steps=20):
     return table[best]["block_size"], table[best]["lanes_per_particle"]
 
 
+def fixed_shape(args):
+    """The build launch shape given by --shape, or None."""
+    return tuple(int(x) for x in args.shape.split(",")) if args.shape else None
+
+
 def small_configs(args, nat):
     """The two small stated systems on the device (extra key `small_configs`),
     each from a seeded start, device time of the timed steps from HIP events:
@@ -455,13 +460,13 @@ def small_configs(args, nat):
     state, with the mixed form's pair-force error there.  Both are
     latency-bound (a few microseconds of launch and dependency per kernel):
     the rate is what one small system gets alone.  The build launch shape of
-    each is the one sc_autotune picks on a copy of the system.  Returns the
+    each is the one sc_autotune picks on a copy of the system (or --shape).  Returns the
     results and the states the CPU baseline times on."""
     from simulationcode_amd import workloads as W
     out, states = {}, {}
     p, D, r, v = W.replica(1000, 3, potential="lj", system="newtonian", kBT=0.1, dt=0.01, rho=1.0,
                            precision="fp64", seed=args.seed + 1, cell_divisor=0)
-    shape = tuned_shape(nat, p, D, r, v)
+    shape = fixed_shape(args) or tuned_shape(nat, p, D, r, v)
     ctx = nat.Context(p, D)
     ctx.set_build_launch(*shape)
     ctx.set_state(r=r, v=v)
@@ -487,7 +492,7 @@ def small_configs(args, nat):
             ctx.update_neighbor_lists()
             ctx.minimize(200, 0.005, 0.02)
             start = ctx.get_state(F=False)["r"]
-            shape = tuned_shape(nat, p, D, start, v)
+            shape = fixed_shape(args) or tuned_shape(nat, p, D, start, v)
         ctx.set_build_launch(*shape)
         # pair forces on the common state: zero velocities and injected zero
         # noise, so that the Langevin terms vanish in both forms
@@ -573,6 +578,20 @@ def batched_summary(args, nat, steps=300, warmup=20):
     return res
 
 
+def bench_small(args, nat):
+    """The stated small systems alone (the `small_configs` of the default
+    run): LJ N=1000 Newtonian as the line's value, the N=8000 Langevin pair
+    beside it."""
+    small, _ = small_configs(args, nat)
+    lj = small["lj_N1000_newtonian_fp64"]
+    return {"metric": "particle-steps/s", "value": lj["value"], "unit": "particle-steps/s", "n_gpus": 1,
+            "steps": lj["steps"], "warmup": 0, "ms_per_step": lj["us_per_step"] / 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded lattices)",
+            "config": {"workload": "3D Lennard-Jones N=1000, rho=1, Newtonian dt=0.01 from kBT=0.1; 3D "
+                                   "polydisperse N=8000, rho=1.02, Langevin kBT=0.15 dt=0.01 (FP64 and mixed)"},
+            "small_configs": small, "roofline": None, "cpu_baseline": None}
+
+
 def bench_batched(args, nat):
     from simulationcode_amd import workloads as W
     specs = W.batched_specs(args.replicas)
@@ -605,8 +624,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--N", type=int, default=10 ** 6)
-    ap.add_argument("--workload", choices=("large", "batched", "served", "copycal"), default="large",
-                    help="copycal: five plain 256 MiB device copies, the reference reading for HBM counters")
+    ap.add_argument("--workload", choices=("large", "batched", "served", "small", "copycal"), default="large",
+                    help="small: the stated small systems alone; copycal: five plain 256 MiB device copies, "
+                         "the reference reading for HBM counters")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-copy-ceiling", action="store_true", help="large: skip the measured device-copy ceiling")
@@ -625,7 +645,8 @@ def main():
     ap.add_argument("--cell-divisor", type=int, default=0,
                     help="0: library default (large) / autotuned (batched)")
     ap.add_argument("--shape", default="",
-                    help="build launch shape BS,LPP (large: both contexts; batched: with --cell-divisor)")
+                    help="build launch shape BS,LPP (large: both contexts and the small systems; small: both "
+                         "systems; batched: with --cell-divisor)")
     ap.add_argument("--build-mode", type=int, default=0, help="large: sc_params.build_mode (0 automatic)")
     ap.add_argument("--chunk", type=int, default=12,
                     help="batched: steps launched ahead per chunk (0: one launch sequence per step)")
@@ -652,6 +673,8 @@ def main():
         out = bench_large(args, nat, synth)
     elif args.workload == "batched":
         out = bench_batched(args, nat)
+    elif args.workload == "small":
+        out = bench_small(args, nat)
     else:
         from simulationcode_amd import serve
         out = serve.bench(args, log)
