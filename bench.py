@@ -106,6 +106,8 @@ def drift_run(ctx, steps, samples=40):
         ctx.do_time_steps(chunk)
         t.append((k + 1) * chunk)
         e.append(total_energy(ctx))
+        if steps >= 200000:  # a long window reports as it goes
+            log(f"[bench] drift window: {t[-1]} of {steps} steps, E = {e[-1]:.10g}")
     slope = np.polyfit(np.array(t, float), np.array(e), 1)[0]
     return float(slope * 1e5 / abs(e[0])), float((max(e) - min(e)) / abs(e[0]))
 
